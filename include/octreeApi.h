@@ -1,0 +1,136 @@
+/*
+ * octreeApi.h - C ABI of the MI355X LOD octree builder (libOctreeApi.so).
+ *
+ * Drop-in for the reference's session API (OctreeApi/include/octreeApi.h:42-72): the same 18
+ * functions with the same signatures, argument meaning and return codes (SUCCESS 0 /
+ * UNEXPECTED_ERROR -1, octreeApi.h:37-38). The entry points after the first block are additive;
+ * each says what it is for.
+ *
+ * Behavioural differences from the reference, all on paths where the reference crashes, exits
+ * or is undefined:
+ *   - no call ever terminates the process: HIP failures, invalid configurations and call-order
+ *     violations return -1; ocpi_get_last_error() gives the reason;
+ *   - the configuration is validated in ocpi_init_octree (chunking grid a power of two in
+ *     [2, 512], intra-cell + weighted rejected, non-empty cloud, E = floor(N * factor) >= N);
+ *   - ocpi_generate_octree returns -1 when the subsampled points do not fit the estimated output
+ *     (N + I > E), where the reference writes past its buffers;
+ *   - ocpi_generate_octree may be called again on the same session (a rebuild of the same cloud);
+ *   - every session owns its device and stream, so separate sessions may be used from separate
+ *     threads.
+ */
+#ifndef OCTREE_API_H
+#define OCTREE_API_H
+
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+#define EXPORTED extern "C" __attribute__((visibility("default")))
+#else
+#define EXPORTED __attribute__((visibility("default")))
+#endif
+
+#define SUCCESS          (0)
+#define UNEXPECTED_ERROR (-1)
+
+/* ---- the reference's API (OctreeApi/include/octreeApi.h:42-72) -------------------------------- */
+
+/* octreeApi.h:42 - creates a session bound to HIP device `device` (session.cpp:27-46). */
+EXPORTED int ocpi_create_session (void** session, int device);
+/* octreeApi.h:43 - frees the session and every device buffer it owns. */
+EXPORTED int ocpi_destroy_session (void* session);
+/* octreeApi.h:44 - 1 info, 2 warn, 3 error, anything else debug (octreeApi.cpp:23-42). */
+EXPORTED int ocpi_set_logging_level (int level);
+
+/* octreeApi.h:46 - 0 = 3 x float32 + 3 x uint8, 1 = 3 x float64 + 3 x uint8 (metadata.cuh:25-29). */
+EXPORTED int ocpi_set_cloud_type (void* session, uint8_t cloudType);
+/* octreeApi.h:47 */
+EXPORTED int ocpi_set_cloud_point_amount (void* session, uint32_t pointAmount);
+/* octreeApi.h:48 - record stride in bytes; the colour follows the three coordinates. */
+EXPORTED int ocpi_set_cloud_data_stride (void* session, uint32_t dataStride);
+/* octreeApi.h:49 - quantisation scale of the exported int32 positions. */
+EXPORTED int ocpi_set_cloud_scale (void* session, double x, double y, double z);
+/* octreeApi.h:50 - stored; like the reference it does not change the output. */
+EXPORTED int ocpi_set_cloud_offset (void* session, double x, double y, double z);
+/* octreeApi.h:51-52 - the cubic bounding box (equal sides, max = min + side). */
+EXPORTED int ocpi_set_cloud_bb (
+        void* session, double minX, double minY, double minZ, double maxX, double maxY, double maxZ);
+
+/* octreeApi.h:54 - borrowed host buffer of pointAmount * stride bytes; copied to HBM by
+ * ocpi_init_octree, so it must stay valid until that call returns. */
+EXPORTED int ocpi_set_point_cloud_host (void* session, uint8_t* pointCloud);
+
+/* octreeApi.h:56-57 */
+EXPORTED int ocpi_configure_chunking (
+        void* session, uint32_t chunkingGrid, uint32_t mergingThreshold, float outputFactor);
+/* octreeApi.h:58-65 - useRandomSubsampling selects explicit random subsampling, otherwise the
+ * implicit (first point) method; replacementScheme only sets a metadata flag. */
+EXPORTED int ocpi_configure_subsampling (
+        void* session,
+        uint32_t subsamplingGrid,
+        bool intraCellAveraging,
+        bool interCellAveraging,
+        bool replacementScheme,
+        bool useRandomSubsampling,
+        bool useWeightingFunction);
+
+/* octreeApi.h:67 - records the file name; the report is written when the session is destroyed. */
+EXPORTED int ocpi_export_memory_report (void* session, const char* filename);
+/* octreeApi.h:68 - leaf point-count histogram as HTML; binWidth 0 picks Scott's rule. */
+EXPORTED int ocpi_export_distribution_histogram (void* session, const char* filename, uint32_t binWidth);
+/* octreeApi.h:69 - statistics.json. */
+EXPORTED int ocpi_export_json_report (void* session, const char* filename);
+/* octreeApi.h:70 - writes hierarchy.bin, octree.bin and metadata.json into an existing directory. */
+EXPORTED int ocpi_export_potree (void* session, const char* directory);
+/* octreeApi.h:71 - validates the configuration, allocates HBM and uploads the cloud. */
+EXPORTED int ocpi_init_octree (void* session);
+/* octreeApi.h:72 - counting, merging, distribution and subsampling on the GPU. */
+EXPORTED int ocpi_generate_octree (void* session);
+
+/* ---- additive entry points ------------------------------------------------------------------- */
+
+/* Borrowed device buffer (HBM) of pointAmount * stride bytes, used in place by init and generate.
+ * The reference has this cloud memory type (CloudMemory ClOUD_DEVICE, metadata.cuh:34-38,
+ * point_cloud.cu:24-44) but no setter for it. */
+EXPORTED int ocpi_set_point_cloud_device (void* session, uint8_t* devicePointCloud);
+
+/* Seed of the explicit random subsampling. The reference seeds with std::time
+ * (octree_processor_impl.cu:82); here the selection is a reproducible function of the seed. */
+EXPORTED int ocpi_set_random_seed (void* session, uint64_t seed);
+
+/* Message of the last failing call on this thread ("" if none). Returns the message length. */
+EXPORTED int ocpi_get_last_error (char* buffer, uint32_t bufferSize);
+
+/* Sizes of the last build. summary receives up to n values:
+ * [0] sparse nodes M, [1] points N, [2] output capacity E, [3] used output records N + I,
+ * [4] internal nodes, [5] leaf nodes, [6] depth, [7] dense nodes D, [8] exported (hierarchy) nodes */
+EXPORTED int ocpi_get_build_summary (void* session, uint64_t* summary, uint32_t n);
+
+/* Copies the M sparse nodes of the last build (ids in canonical order: level ascending from the
+ * finest, then linear index). flags: bit0 finished, bit1 internal. children: 8 ints per node,
+ * slot = 4 * dx + 2 * dy + dz. Any pointer may be NULL. */
+EXPORTED int ocpi_copy_nodes (
+        void* session,
+        uint32_t* denseKey,
+        uint32_t* denseCount,
+        uint32_t* pointCount,
+        uint64_t* dataIdx,
+        int32_t* parent,
+        int32_t* children,
+        uint8_t* flags);
+
+/* Copies the first `count` entries of the point LUT (source point index per output record). */
+EXPORTED int ocpi_copy_point_lut (void* session, uint32_t* destination, uint64_t count);
+
+/* Copies the first `count` 18-byte output records (int32 x, y, z; uint16 r, g, b). */
+EXPORTED int ocpi_copy_output (void* session, uint8_t* destination, uint64_t count);
+
+/* Device-side durations of the last ocpi_generate_octree, in milliseconds, measured with HIP
+ * events on the session stream: [0] counting kernel, [1] merge (pyramid, nodes, leaf blocks),
+ * [2] distribution kernel, [3] subsampling (all levels and the final placement), [4] whole
+ * build, [5] summed subsampling insert kernels, [6] summed subsampling emit kernels. */
+EXPORTED int ocpi_get_stage_timings (void* session, float* milliseconds, uint32_t n);
+
+#endif

@@ -1,0 +1,190 @@
+// OctreeBuilder: host orchestration of the level-batched GPU build on one HIP stream.
+//
+// Replaces the reference's OctreeProcessorImpl (octree_processor_impl.cu:12-186,
+// octree_chunking.cu:18-124, rp_subsampling.cu, fp_subsampling.cu). Phases:
+//   count       one pass over the cloud: dense level-0 histogram (K1)
+//   merge       count pyramid, canonical sparse ids by scan, node links and flags, leaf data
+//               blocks by scan (K2-K4), cell -> leaf map
+//   distribute  one pass over the cloud: slot per point in its leaf, point index, 18-byte
+//               record and the coordinates for the subsampling (K5)
+//   subsample   one batch per octree level (all internal nodes of a level at once) with per-node
+//               open-addressing cell tables; winners by atomicMin, colours by packed integer
+//               sums or a gathered distance-weighted filter; then DFS post-order placement
+// All buffers live in a grow-only per-session arena, so repeated builds allocate nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "types.h"
+
+namespace oc {
+
+struct CloudInfo  // PointCloudInfo (metadata.cuh:52-71)
+{
+    uint32_t pointAmount = 0;
+    uint32_t dataStride  = 0;
+    double bbMin[3]      = {0, 0, 0};
+    double bbMax[3]      = {0, 0, 0};
+    double offset[3]     = {0, 0, 0};
+    double scale[3]      = {1, 1, 1};
+    uint8_t cloudType    = 0;
+};
+
+struct ProcessingInfo  // ProcessingInfo (metadata.cuh:75-88)
+{
+    uint32_t subsamplingGrid  = 128;
+    uint32_t chunkingGrid     = 512;
+    uint32_t mergingThreshold = 10000;
+    bool useIntraCellAvg      = false;
+    bool useInterCellAvg      = false;
+    bool useWeightingFunction = false;
+    bool useReplacementScheme = false;
+    bool useRandomSubsampling = false;
+    float outputFactor        = 2.2f;
+    uint64_t seed             = 0;
+};
+
+// Host copy of the finished octree (for export and reports).
+struct HostOctree
+{
+    std::vector<uint32_t> denseKey, count, pointCount;
+    std::vector<uint64_t> dataIdx;
+    std::vector<int32_t> parent, children;
+    std::vector<uint8_t> flags;
+};
+
+class DeviceBuffer
+{
+public:
+    DeviceBuffer() = default;
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+
+    // Grows (never shrinks) to at least `bytes`; contents are not preserved on growth.
+    void ensure(size_t bytes);
+    void release();
+    void* get() const { return itsPtr; }
+    size_t capacity() const { return itsBytes; }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(itsPtr);
+    }
+
+    static size_t totalAllocated();
+    static size_t peakAllocated();
+
+private:
+    void* itsPtr   = nullptr;
+    size_t itsBytes = 0;
+};
+
+class OctreeBuilder
+{
+public:
+    OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
+                  const uint8_t* deviceCloud);
+    ~OctreeBuilder();
+    OctreeBuilder(const OctreeBuilder&) = delete;
+    OctreeBuilder& operator=(const OctreeBuilder&) = delete;
+
+    void generate();
+
+    bool built() const { return itsBuilt; }
+    const CloudInfo& cloud() const { return itsCloud; }
+    const ProcessingInfo& processing() const { return itsProc; }
+    uint32_t depth() const { return itsDepth; }
+    uint32_t denseNodes() const { return itsDense; }
+    uint32_t sparseNodes() const { return itsM; }
+    uint64_t capacity() const { return itsE; }
+    uint64_t used() const { return itsUsed; }
+    uint32_t levelStart(uint32_t level) const { return itsLevelStart[level]; }
+
+    // Copies device results to host (valid after generate()).
+    void copyNodes(HostOctree& out) const;
+    void copyLut(uint32_t* dst, uint64_t count) const;
+    void copyOutput(uint8_t* dst, uint64_t count) const { copyOutputRange(dst, 0, count); }
+    void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const;
+    const std::vector<float>& stageTimings() const { return itsTimings; }
+
+private:
+    template <typename T>
+    void generateTyped();
+    template <typename T>
+    void countPoints();
+    void mergeLevels();
+    template <typename T>
+    void distributePoints();
+    template <typename T>
+    void subsampleLevels();
+    void placeInternalNodes();
+
+    void syncStream();
+    template <typename V>
+    V readDevice(const V* devicePtr);
+
+    int itsDevice;
+    hipStream_t itsStream = nullptr;
+    CloudInfo itsCloud;
+    ProcessingInfo itsProc;
+    ColorMode itsColorMode = COLOR_NONE;
+
+    // level tables (octree.cu:13-23)
+    uint32_t itsDepth = 0;
+    uint32_t itsDense = 0;
+    uint32_t itsLevelGrid[MAX_LEVELS]   = {};
+    uint32_t itsLevelOffset[MAX_LEVELS] = {};
+    uint32_t itsLevelNodes[MAX_LEVELS]  = {};
+
+    // results of the last build
+    bool itsBuilt          = false;
+    uint32_t itsM          = 0;
+    uint64_t itsE          = 0;
+    uint64_t itsUsed       = 0;
+    uint32_t itsInternal   = 0;
+    uint32_t itsLevelStart[MAX_LEVELS + 1] = {};
+    std::vector<float> itsTimings;
+
+    // cloud
+    const uint8_t* itsCloudDevice = nullptr;
+    DeviceBuffer itsCloudOwned;
+
+    // dense pyramid and sparse nodes
+    DeviceBuffer itsCount, itsDenseToSparse;
+    DeviceBuffer itsNodeKey, itsNodeCount, itsNodePoints, itsNodeData, itsNodeParent, itsNodeChild, itsNodeFlags,
+            itsNodeFill, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
+
+    // output region [0, E): leaf points, then staged internal subsamples
+    DeviceBuffer itsLut, itsOut, itsPts, itsStageNode, itsTmpLut, itsTmpOut;
+
+    // per-level subsampling workspace
+    DeviceBuffer itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid, itsNodeCap, itsTableBase, itsTileList;
+    DeviceBuffer itsKeys, itsWin, itsColor, itsCellCnt, itsCellStart, itsCellFill, itsSlot;
+    DeviceBuffer itsEntryOf, itsPosOf, itsCsr;
+
+    // small device scalars and scan scratch
+    DeviceBuffer itsScalars, itsScanScratch;
+
+    enum EventSlot
+    {
+        EV_START,
+        EV_COUNT_BEGIN,
+        EV_COUNT_END,
+        EV_MERGE_END,
+        EV_DIST_BEGIN,
+        EV_DIST_END,
+        EV_END,
+        EV_SLOTS
+    };
+    hipEvent_t itsEvents[EV_SLOTS]              = {};
+    hipEvent_t itsLevelEvents[MAX_LEVELS][4]    = {};  // insert begin/end, emit begin/end per level
+    uint32_t itsLevelsRun                       = 0;
+};
+
+}  // namespace oc

@@ -1,0 +1,173 @@
+// Device-side helpers shared by the builder's kernels: record decoding, the cell mapping, packed
+// colours, the explicit-random ranking and the per-node open-addressing tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "types.h"
+
+namespace oc {
+
+// Unaligned load of the three coordinates of a packed AoS record (the reference reads them through
+// a #pragma pack(1) Vector3, metadata.cuh:13-19). gfx950 serves unaligned dword loads directly.
+template <typename T>
+__device__ __forceinline__ void loadXYZ(const uint8_t* rec, double& x, double& y, double& z)
+{
+    T v[3];
+    __builtin_memcpy(v, rec, sizeof(v));
+    x = static_cast<double>(v[0]);
+    y = static_cast<double>(v[1]);
+    z = static_cast<double>(v[2]);
+}
+
+// Cell coordinates of a point in a grid, with the reference's exact expression order
+// (kernel_helpers.cuh:11-25): t = bbSize / G; u = (p - min) / t; clamp to G - 1 with fmin;
+// truncate to int64, then reinterpret as uint64.
+__device__ __forceinline__ void gridCoords(double px, double py, double pz, const Gridding& g, uint64_t& ix,
+                                           uint64_t& iy, uint64_t& iz)
+{
+    double t  = g.bbSize / g.gridSize;
+    double uX = (px - g.minX) / t;
+    double uY = (py - g.minY) / t;
+    double uZ = (pz - g.minZ) / t;
+    t         = g.gridSize - 1.0;
+    ix        = static_cast<uint64_t>(static_cast<int64_t>(fmin(uX, t)));
+    iy        = static_cast<uint64_t>(static_cast<int64_t>(fmin(uY, t)));
+    iz        = static_cast<uint64_t>(static_cast<int64_t>(fmin(uZ, t)));
+}
+
+__device__ __forceinline__ uint32_t mapPointToGrid(double px, double py, double pz, const Gridding& g)
+{
+    uint64_t ix, iy, iz;
+    gridCoords(px, py, pz, g, ix, iy, iz);
+    return static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
+}
+
+// Packed colour (kernel_helpers.cuh:49-53): 18-bit r, g, b fields and a 10-bit count; sums wrap
+// exactly like the reference's uint64 atomics.
+__device__ __forceinline__ uint64_t encodeColors(uint16_t r, uint16_t g, uint16_t b)
+{
+    return (static_cast<uint64_t>(r) << 46) | (static_cast<uint64_t>(g) << 28) | (static_cast<uint64_t>(b) << 10) |
+           1ull;
+}
+
+// rp_subsampling.cuh:123-129; a zero amount (count == 0 mod 1024) decodes to 0 instead of dividing
+// by zero.
+__device__ __forceinline__ void decodeColors(uint64_t e, uint16_t& r, uint16_t& g, uint16_t& b)
+{
+    uint32_t amount = static_cast<uint32_t>(e & 0x3FF);
+    if (amount == 0)
+    {
+        r = g = b = 0;
+        return;
+    }
+    r = static_cast<uint16_t>(static_cast<uint32_t>((e >> 46) & 0xFFFF) / amount);
+    g = static_cast<uint16_t>(static_cast<uint32_t>((e >> 28) & 0xFFFF) / amount);
+    b = static_cast<uint16_t>(static_cast<uint32_t>((e >> 10) & 0xFFFF) / amount);
+}
+
+// Explicit random subsampling ranks the points of a node with MurmurHash3's 32-bit finalizer
+// (a bijection, so no ties) keyed by a per-node salt from the SplitMix64 finalizer; the winner of
+// a cell is the point of minimum rank. Identical on host and device.
+__host__ __device__ __forceinline__ uint64_t splitmix64Mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6BU;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35U;
+    h ^= h >> 16;
+    return h;
+}
+
+__host__ __device__ __forceinline__ uint32_t nodeSalt(uint64_t seed, uint32_t nodeKey)
+{
+    return static_cast<uint32_t>(
+            splitmix64Mix(seed + 0x9E3779B97F4A7C15ULL * (static_cast<uint64_t>(nodeKey) + 1)) >> 32);
+}
+
+// ---- per-node open-addressing cell tables ----------------------------------------------------------
+//
+// Each internal node of a level owns a contiguous slice [base, base + cap) of the level's table
+// with cap > 2 * (its input points), so a probe always reaches an empty slot and every loop
+// below terminates. Keys are subsampling-cell indices (< 2^30), EMPTY_KEY marks a free slot.
+
+constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t tableHome(uint32_t cell, uint32_t cap)
+{
+    return static_cast<uint32_t>((static_cast<uint64_t>(fmix32(cell)) * cap) >> 32);
+}
+
+// Returns the slot of `cell`, inserting it if absent; *inserted tells whether this call claimed it.
+__device__ __forceinline__ uint32_t tableInsert(uint32_t* keys, uint32_t cap, uint32_t cell, bool& inserted)
+{
+    uint32_t i = tableHome(cell, cap);
+    inserted   = false;
+    for (;;)
+    {
+        uint32_t k = __hip_atomic_load(keys + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == cell)
+            return i;
+        if (k == EMPTY_KEY)
+        {
+            uint32_t prev = atomicCAS(keys + i, EMPTY_KEY, cell);
+            if (prev == EMPTY_KEY)
+            {
+                inserted = true;
+                return i;
+            }
+            if (prev == cell)
+                return i;
+        }
+        i = (i + 1 == cap) ? 0 : i + 1;
+    }
+}
+
+// Returns the slot of `cell` or EMPTY_KEY when absent (keys are stable while this is called).
+__device__ __forceinline__ uint32_t tableFind(const uint32_t* keys, uint32_t cap, uint32_t cell)
+{
+    uint32_t i = tableHome(cell, cap);
+    for (;;)
+    {
+        uint32_t k = keys[i];
+        if (k == cell)
+            return i;
+        if (k == EMPTY_KEY)
+            return EMPTY_KEY;
+        i = (i + 1 == cap) ? 0 : i + 1;
+    }
+}
+
+__device__ __forceinline__ void storeRecord(OutRecord* dst, const OutRecord& r)
+{
+    __builtin_memcpy(dst, &r, sizeof(OutRecord));
+}
+
+__device__ __forceinline__ OutRecord loadRecord(const OutRecord* src)
+{
+    OutRecord r;
+    __builtin_memcpy(&r, src, sizeof(OutRecord));
+    return r;
+}
+
+// grid-stride helpers
+__device__ __forceinline__ uint64_t globalThread()
+{
+    return static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ uint64_t globalStride()
+{
+    return static_cast<uint64_t>(gridDim.x) * blockDim.x;
+}
+
+}  // namespace oc

@@ -1,0 +1,250 @@
+// The C ABI (include/octreeApi.h). Every entry point runs inside execute(), which maps any
+// exception to UNEXPECTED_ERROR (-1) like the reference's Execute wrapper (octreeApi.cpp:9-21)
+// and keeps the message for ocpi_get_last_error; HIP failures never terminate the process.
+#include "octreeApi.h"
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "log.h"
+#include "reports.h"
+#include "potree_writer.h"
+#include "session.h"
+
+namespace {
+
+thread_local std::string tLastError;
+
+template <typename F>
+int execute(F&& f)
+{
+    try
+    {
+        f();
+        tLastError.clear();
+        return SUCCESS;
+    }
+    catch (const std::exception& e)
+    {
+        tLastError = e.what();
+    }
+    catch (...)
+    {
+        tLastError = "unknown error";
+    }
+    oc::log(oc::LOG_ERROR, tLastError);
+    return UNEXPECTED_ERROR;
+}
+
+oc::Session* S(void* h)
+{
+    return oc::Session::fromHandle(h);
+}
+
+}  // namespace
+
+int ocpi_set_logging_level(int level)
+{
+    return execute([&] {
+        switch (level)  // octreeApi.cpp:26-40
+        {
+        case 1: oc::setLogLevel(oc::LOG_INFO); break;
+        case 2: oc::setLogLevel(oc::LOG_WARN); break;
+        case 3: oc::setLogLevel(oc::LOG_ERROR); break;
+        default: oc::setLogLevel(oc::LOG_DEBUG); break;
+        }
+    });
+}
+
+int ocpi_create_session(void** session, int device)
+{
+    return execute([&] {
+        if (!session)
+            throw std::runtime_error("null session pointer");
+        *session = nullptr;
+        *session = new oc::Session(device);
+    });
+}
+
+int ocpi_destroy_session(void* session)
+{
+    return execute([&] { delete S(session); });
+}
+
+int ocpi_set_point_cloud_host(void* session, uint8_t* pointCloud)
+{
+    return execute([&] { S(session)->setPointCloudHost(pointCloud); });
+}
+
+int ocpi_set_point_cloud_device(void* session, uint8_t* devicePointCloud)
+{
+    return execute([&] { S(session)->setPointCloudDevice(devicePointCloud); });
+}
+
+int ocpi_init_octree(void* session)
+{
+    return execute([&] { S(session)->initOctree(); });
+}
+
+int ocpi_generate_octree(void* session)
+{
+    return execute([&] { S(session)->generateOctree(); });
+}
+
+int ocpi_export_potree(void* session, const char* directory)
+{
+    return execute([&] {
+        if (!directory)
+            throw std::runtime_error("null directory");
+        S(session)->exportPotree(directory);
+    });
+}
+
+int ocpi_configure_chunking(void* session, uint32_t chunkingGrid, uint32_t mergingThreshold, float outputFactor)
+{
+    return execute([&] { S(session)->configureChunking(chunkingGrid, mergingThreshold, outputFactor); });
+}
+
+int ocpi_configure_subsampling(void* session, uint32_t subsamplingGrid, bool intraCellAveraging,
+                               bool interCellAveraging, bool replacementScheme, bool useRandomSubsampling,
+                               bool useWeightingFunction)
+{
+    return execute([&] {
+        S(session)->configureSubsampling(subsamplingGrid, intraCellAveraging, interCellAveraging, replacementScheme,
+                                         useRandomSubsampling, useWeightingFunction);
+    });
+}
+
+int ocpi_export_memory_report(void* session, const char* filename)
+{
+    return execute([&] {
+        if (!filename)
+            throw std::runtime_error("null filename");
+        S(session)->exportMemoryReport(filename);
+    });
+}
+
+int ocpi_export_json_report(void* session, const char* filename)
+{
+    return execute([&] {
+        if (!filename)
+            throw std::runtime_error("null filename");
+        S(session)->exportJsonReport(filename);
+    });
+}
+
+int ocpi_export_distribution_histogram(void* session, const char* filename, uint32_t binWidth)
+{
+    return execute([&] {
+        if (!filename)
+            throw std::runtime_error("null filename");
+        S(session)->exportDistributionHistogram(filename, binWidth);
+    });
+}
+
+int ocpi_set_cloud_type(void* session, uint8_t cloudType)
+{
+    return execute([&] { S(session)->setCloudType(cloudType); });
+}
+
+int ocpi_set_cloud_point_amount(void* session, uint32_t pointAmount)
+{
+    return execute([&] { S(session)->setCloudPointAmount(pointAmount); });
+}
+
+int ocpi_set_cloud_data_stride(void* session, uint32_t dataStride)
+{
+    return execute([&] { S(session)->setCloudDataStride(dataStride); });
+}
+
+int ocpi_set_cloud_scale(void* session, double x, double y, double z)
+{
+    return execute([&] { S(session)->setCloudScale(x, y, z); });
+}
+
+int ocpi_set_cloud_offset(void* session, double x, double y, double z)
+{
+    return execute([&] { S(session)->setCloudOffset(x, y, z); });
+}
+
+int ocpi_set_cloud_bb(void* session, double minX, double minY, double minZ, double maxX, double maxY, double maxZ)
+{
+    return execute([&] { S(session)->setCloudBoundingBox(minX, minY, minZ, maxX, maxY, maxZ); });
+}
+
+int ocpi_set_random_seed(void* session, uint64_t seed)
+{
+    return execute([&] { S(session)->setRandomSeed(seed); });
+}
+
+int ocpi_get_last_error(char* buffer, uint32_t bufferSize)
+{
+    const std::string& e = tLastError;
+    if (buffer && bufferSize > 0)
+    {
+        size_t n = std::min<size_t>(e.size(), bufferSize - 1);
+        std::memcpy(buffer, e.data(), n);
+        buffer[n] = '\0';
+    }
+    return static_cast<int>(e.size());
+}
+
+int ocpi_get_build_summary(void* session, uint64_t* summary, uint32_t n)
+{
+    return execute([&] {
+        oc::Session* s          = S(session);
+        oc::OctreeBuilder& b    = s->builder();
+        const oc::HostOctree& t = s->hostOctree();
+        oc::OctreeInfo info     = oc::nodeStatistics(b, t);
+        uint64_t exported       = 0;
+        uint64_t nodes          = oc::hierarchyEntries(t, &exported).size();
+        uint64_t v[9] = {b.sparseNodes(), b.cloud().pointAmount, b.capacity(), b.used(),
+                         info.internalNodeAmount, info.leafNodeAmount, b.depth(), b.denseNodes(), nodes};
+        for (uint32_t i = 0; i < n && i < 9; ++i)
+            summary[i] = v[i];
+    });
+}
+
+int ocpi_copy_nodes(void* session, uint32_t* denseKey, uint32_t* denseCount, uint32_t* pointCount, uint64_t* dataIdx,
+                    int32_t* parent, int32_t* children, uint8_t* flags)
+{
+    return execute([&] {
+        const oc::HostOctree& t = S(session)->hostOctree();
+        const size_t m          = t.flags.size();
+        if (denseKey)
+            std::copy(t.denseKey.begin(), t.denseKey.end(), denseKey);
+        if (denseCount)
+            std::copy(t.count.begin(), t.count.end(), denseCount);
+        if (pointCount)
+            std::copy(t.pointCount.begin(), t.pointCount.end(), pointCount);
+        if (dataIdx)
+            std::copy(t.dataIdx.begin(), t.dataIdx.end(), dataIdx);
+        if (parent)
+            std::copy(t.parent.begin(), t.parent.end(), parent);
+        if (children)
+            std::copy(t.children.begin(), t.children.begin() + 8 * m, children);
+        if (flags)
+            std::copy(t.flags.begin(), t.flags.end(), flags);
+    });
+}
+
+int ocpi_copy_point_lut(void* session, uint32_t* destination, uint64_t count)
+{
+    return execute([&] { S(session)->builder().copyLut(destination, count); });
+}
+
+int ocpi_copy_output(void* session, uint8_t* destination, uint64_t count)
+{
+    return execute([&] { S(session)->builder().copyOutput(destination, count); });
+}
+
+int ocpi_get_stage_timings(void* session, float* milliseconds, uint32_t n)
+{
+    return execute([&] {
+        const auto& t = S(session)->builder().stageTimings();
+        for (uint32_t i = 0; i < n; ++i)
+            milliseconds[i] = i < t.size() ? t[i] : 0.f;
+    });
+}

@@ -1,0 +1,159 @@
+// Device-wide exclusive prefix sum for the builder (sparse ids, leaf blocks, table slices, slots).
+//
+// Reduce-then-scan in three launches with no inter-workgroup waiting: (1) each workgroup reduces
+// its tiles of SCAN_TILE items, (2) one workgroup scans the tile sums, (3) each workgroup rescans
+// its tiles with the tile offset and hands (index, exclusive prefix, value) to an output functor.
+// Inside a workgroup the scan is a wave64 shuffle scan (6 steps) plus one LDS step across the
+// four waves. Inputs and outputs are functors so the flags / values are generated and consumed in
+// the same pass as the scan (e.g. occupancy flags straight from the counting grid).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace oc {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS   = 8;
+constexpr int SCAN_TILE    = SCAN_THREADS * SCAN_ITEMS;
+constexpr int SCAN_WAVES   = SCAN_THREADS / 64;
+
+template <typename T>
+__device__ __forceinline__ T waveInclusiveScan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        T n = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += n;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread over the workgroup; *total gets the workgroup sum.
+template <typename T>
+__device__ __forceinline__ T blockExclusiveScan(T v, T& total, T* lds)
+{
+    T inc        = waveInclusiveScan(v);
+    const int w  = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63)
+        lds[w] = inc;
+    __syncthreads();
+    T off = 0, sum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_WAVES; ++i)
+    {
+        T s = lds[i];
+        off += (i < w) ? s : T(0);
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return inc - v + off;
+}
+
+template <typename T, typename In>
+__global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* tileSums, uint64_t tiles)
+{
+    __shared__ T lds[SCAN_WAVES];
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        uint64_t base = tile * SCAN_TILE + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
+        T s           = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            if (base + k < n)
+                s += in(base + k);
+        T total;
+        (void)blockExclusiveScan(s, total, lds);
+        if (threadIdx.x == 0)
+            tileSums[tile] = total;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t tiles, T* total)
+{
+    __shared__ T lds[SCAN_WAVES];
+    T carry = 0;
+    for (uint64_t base = 0; base < tiles; base += SCAN_TILE)
+    {
+        uint64_t first = base + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            v[k] = (first + k < tiles) ? tileSums[first + k] : T(0);
+            s += v[k];
+        }
+        T chunk;
+        T off = blockExclusiveScan(s, chunk, lds) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            if (first + k < tiles)
+                tileSums[first + k] = off;
+            off += v[k];
+        }
+        carry += chunk;
+    }
+    if (threadIdx.x == 0 && total)
+        *total = carry;
+}
+
+template <typename T, typename In, typename Out>
+__global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* tileOffsets,
+                                                          uint64_t tiles)
+{
+    __shared__ T lds[SCAN_WAVES];
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    {
+        uint64_t base = tile * SCAN_TILE + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            v[k] = (base + k < n) ? in(base + k) : T(0);
+            s += v[k];
+        }
+        T total;
+        T off = blockExclusiveScan(s, total, lds) + tileOffsets[tile];
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            if (base + k < n)
+                out(base + k, off, v[k]);
+            off += v[k];
+        }
+    }
+}
+
+inline uint64_t scanTiles(uint64_t n)
+{
+    return (n + SCAN_TILE - 1) / SCAN_TILE;
+}
+
+// Exclusive scan of in(0..n-1); out(i, prefix, value) for each i; *total (device) = sum.
+// tileScratch must hold scanTiles(n) values.
+template <typename T, typename In, typename Out>
+void scanExclusive(uint64_t n, In in, Out out, T* tileScratch, T* total, hipStream_t stream)
+{
+    uint64_t tiles = scanTiles(n);
+    if (tiles == 0)
+    {
+        if (total)
+            (void)hipMemsetAsync(total, 0, sizeof(T), stream);
+        return;
+    }
+    unsigned grid = static_cast<unsigned>(tiles < 8192 ? tiles : 8192);
+    kScanReduce<T, In><<<grid, SCAN_THREADS, 0, stream>>>(in, n, tileScratch, tiles);
+    kScanTiles<T><<<1, SCAN_THREADS, 0, stream>>>(tileScratch, tiles, total);
+    kScanDown<T, In, Out><<<grid, SCAN_THREADS, 0, stream>>>(in, out, n, tileScratch, tiles);
+}
+
+}  // namespace oc

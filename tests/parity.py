@@ -1,0 +1,109 @@
+"""Comparison of a GPU build (through the C ABI) with the CPU oracle.
+
+Bit-exact: node keys, dense counts, flags, parents, children, point counts, data offsets of
+every exported node, the used size N + I, the hierarchy bytes, and per node the set of
+(point index, quantised position, colour) records. Record order inside a node is not compared
+(the reference fills leaves and subsamples by atomics). Colours of the distance-weighted filter
+are compared with a tolerance of +-1 per channel, per level, against the oracle fed with the
+GPU's own child colours (weighted_color_check).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+import oracle_binding as O
+from octree_amd import api
+from octree_amd import workloads as W
+
+
+def gpu_build(cloud, n, stride, cloud_type, cubic, scale, spec, device=0, sess=None):
+    own = sess is None
+    s = api.Session(device) if own else sess
+    s.set_cloud(cloud, n, stride, cloud_type, cubic, scale)
+    s.configure(spec)
+    s.init()
+    s.generate()
+    return s
+
+
+def fetch(s):
+    summ = s.summary()
+    nodes = s.nodes()
+    used = summ["used"]
+    lut = s.point_lut(used)
+    out = s.output(used)
+    return summ, nodes, lut, out
+
+
+def node_of_position(data_idx, point_count, mask, used):
+    """Node id owning each output position [0, used) (-1 where none)."""
+    owner = np.full(used, -1, np.int64)
+    ids = np.nonzero(mask)[0]
+    for i in ids:
+        a = int(data_idx[i])
+        owner[a:a + int(point_count[i])] = i
+    return owner
+
+
+def records_by_node(owner, lut, out):
+    keys = np.stack([owner, lut.astype(np.int64)])
+    order = np.lexsort(keys[::-1])
+    cols = np.stack([out[f].astype(np.int64) for f in ("x", "y", "z", "r", "g", "b")], 1)
+    return owner[order], lut[order].astype(np.int64), cols[order]
+
+
+def compare(gpu, orc, color_tol=0, check_colors=True):
+    """Raises AssertionError with a description of the first difference."""
+    summ, nodes, lut, out = gpu
+    assert orc.status == 0, f"oracle failed with status {orc.status}"
+    assert summ["sparse_nodes"] == len(orc.key), (summ["sparse_nodes"], len(orc.key))
+    np.testing.assert_array_equal(nodes.key, orc.key, "node keys")
+    np.testing.assert_array_equal(nodes.count, orc.count, "dense counts")
+    np.testing.assert_array_equal(nodes.flags, orc.flags, "finished/internal flags")
+    np.testing.assert_array_equal(nodes.parent, orc.parent, "parents")
+    np.testing.assert_array_equal(nodes.children, orc.children, "children")
+    np.testing.assert_array_equal(nodes.point_count, orc.point_count, "point counts")
+    fin = orc.finished
+    np.testing.assert_array_equal(nodes.data_idx[fin], orc.data_idx[fin], "data offsets of exported nodes")
+    assert summ["used"] == orc.used, (summ["used"], orc.used)
+    used = orc.used
+    owner = node_of_position(orc.data_idx, orc.point_count, fin, used)
+    assert (owner >= 0).all(), "output positions not covered by exported nodes"
+    g_owner, g_idx, g_cols = records_by_node(owner, lut, out)
+    o_owner, o_idx, o_cols = records_by_node(owner, orc.lut[:used], orc.out[:used])
+    np.testing.assert_array_equal(g_owner, o_owner)
+    np.testing.assert_array_equal(g_idx, o_idx, "per-node point sets")
+    np.testing.assert_array_equal(g_cols[:, :3], o_cols[:, :3], "quantised positions")
+    if check_colors:
+        d = np.abs(g_cols[:, 3:] - o_cols[:, 3:])
+        assert d.max(initial=0) <= color_tol, f"colour difference {d.max()} > {color_tol}"
+
+
+def hierarchy_bytes(s):
+    with tempfile.TemporaryDirectory() as d:
+        s.export_potree(d)
+        with open(os.path.join(d, "hierarchy.bin"), "rb") as f:
+            return f.read()
+
+
+def run_case(cloud, n, stride, cloud_type, cubic, scale, spec, color_tol=0, threads=0):
+    orc = O.build(cloud, n, stride, cloud_type, cubic, scale, spec, threads=threads)
+    s = gpu_build(cloud, n, stride, cloud_type, cubic, scale, spec)
+    try:
+        g = fetch(s)
+        compare(g, orc, color_tol=color_tol)
+        assert hierarchy_bytes(s) == orc.hierarchy, "hierarchy.bin differs"
+        return g, orc
+    finally:
+        s.close()
+
+
+def synthetic(n, cloud_type, box=(20.0, 20.0, 6.0), clusters=4, seed=11, offset=0.0):
+    spec = W.CloudSpec(n, cloud_type, box, clusters, seed, offset=offset)
+    cloud = W.generate(spec)
+    _, cubic = W.bounding_box(cloud, n, spec.stride, cloud_type)
+    return cloud, spec, cubic

@@ -1,0 +1,94 @@
+"""GPU builder (libOctreeApi.so through the C ABI) against the CPU oracle on seeded inputs.
+
+Every case compares node keys, counts, flags, links, point counts, data offsets, the used size,
+hierarchy.bin and the per-node point sets bit-exactly; colours are bit-exact except for the
+distance-weighted filter (float sums in a different order and a different expf: +-1 per channel).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_binding as O
+from octree_amd import workloads as W
+from parity import run_case, synthetic
+
+pytestmark = pytest.mark.gpu
+
+MODES = [  # (intra, inter, weighted, random)
+    (False, False, False, False),
+    (True, False, False, False),
+    (False, True, False, False),
+    (False, True, True, False),
+    (False, False, False, True),
+    (True, False, False, True),
+    (False, True, False, True),
+    (False, True, True, True),
+]
+
+
+def _spec(G, g, T, mode, F=3.0, seed=7):
+    intra, inter, weighted, rnd = mode
+    return W.BuildSpec(G, T, F, g, intra=intra, inter=inter, weighted=weighted, random=rnd, seed=seed)
+
+
+@pytest.mark.parametrize("cloud_type", [0, 1])
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: "-".join(n for n, f in zip(("intra", "inter", "weighted", "random"), m) if f) or "plain")
+def test_small_cloud_all_modes(cloud_type, mode):
+    print(f"case small cloud_type={cloud_type} mode={mode}", flush=True)
+    cloud, spec, cubic = synthetic(60_000, cloud_type, offset=0.0 if cloud_type == 0 else 5e5)
+    tol = 1 if mode[2] else 0
+    run_case(cloud, spec.points, spec.stride, cloud_type, cubic, [spec.scale] * 3, _spec(64, 32, 2000, mode), color_tol=tol)
+
+
+def test_cuboid_known_answers():
+    """test_merging_sparse.cpp / test_subsampling_sparse.cpp on the 128^3 cuboid (15-byte variant)."""
+    print("case cuboid 128^3", flush=True)
+    rec, bb, scale = W.cuboid_cloud(128)
+    spec = W.BuildSpec(128, 10000, 4.2, 128, intra=True)
+    g, orc = run_case(rec, 128 ** 3, 15, 0, bb, scale, spec)
+    summ, nodes, lut, out = g
+    assert summ["sparse_nodes"] == 2_396_745
+    assert nodes.count[-1] == 128 ** 3
+    assert summ["internal_nodes"] == 73
+    # intra-cell averaging of single-point cells passes colours through
+    assert (out["r"][:128 ** 3] == (lut[:128 ** 3] % 128)).all()
+
+
+@pytest.mark.parametrize("G,g,T", [(512, 128, 10000), (256, 64, 5000), (32, 16, 500)])
+def test_grids_and_thresholds(G, g, T):
+    print(f"case grids G={G} g={g} T={T}", flush=True)
+    cloud, spec, cubic = synthetic(300_000, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=3)
+    run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, _spec(G, g, T, (False, True, False, False)))
+
+
+def test_root_as_single_leaf():
+    """N < threshold: the reference walks off the tree (A13); the root becomes the only leaf."""
+    cloud, spec, cubic = synthetic(500, 0)
+    g, orc = run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, _spec(64, 32, 10000, MODES[0]))
+    summ = g[0]
+    assert summ["internal_nodes"] == 0 and summ["leaf_nodes"] == 1 and summ["used"] == 500
+
+
+@pytest.mark.parametrize("n", [1, 2, 7])
+def test_tiny_clouds(n):
+    cloud, spec, cubic = synthetic(max(n, 2), 1)
+    cloud = cloud[: n * spec.stride]
+    _, cubic = W.bounding_box(cloud, n, spec.stride, 1)
+    if cubic[3] - cubic[0] <= 0:
+        cubic = [cubic[0], cubic[1], cubic[2], cubic[0] + 1.0, cubic[1] + 1.0, cubic[2] + 1.0]
+    run_case(cloud, n, spec.stride, 1, cubic, [spec.scale] * 3, _spec(8, 4, 0, MODES[2], F=40.0))
+
+
+def test_threshold_zero_every_node_internal():
+    cloud, spec, cubic = synthetic(5_000, 0)
+    run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, _spec(16, 8, 0, MODES[1], F=16.0))
+
+
+def test_c1_shape_reduced():
+    """The coin-like workload's generator and processing at 1/5 of its size."""
+    wl = W.C1
+    spec = W.CloudSpec(1_000_000, 0, wl.cloud.box, wl.cloud.clusters, wl.cloud.seed)
+    cloud = W.generate(spec)
+    _, cubic = W.bounding_box(cloud, spec.points, spec.stride, 0)
+    run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, wl.build)
