@@ -305,6 +305,7 @@ struct Build
     uint32_t internalNodes    = 0;
     uint32_t referenceBoundBinds = 0;  // A12: internal children the reference's launch bound truncates
     uint32_t maxLeafPoints    = 0;
+    std::vector<double> nodeMins;     // 3 per node, accumulated top-down (tools.cu:29-61)
 };
 
 uint64_t outputCapacity(uint32_t n, float factor)
@@ -347,8 +348,14 @@ int validate(const OracleConfig& c)
 class Builder
 {
 public:
-    Builder(const uint8_t* cloud, const OracleConfig& cfg, Build& b) : itsCloud(cloud), itsB(b)
+    Builder(const uint8_t* cloud, const OracleConfig& cfg, Build& b, bool attach = false) : itsCloud(cloud), itsB(b)
     {
+        if (attach)
+        {
+            itsThreads   = 1;
+            itsCubicSize = cfg.bb_max[0] - cfg.bb_min[0];
+            return;
+        }
         itsB.cfg       = cfg;
         itsB.lt        = levelTables(cfg.chunking_grid);
         itsB.colorMode = resolveColorMode(cfg);
@@ -576,6 +583,7 @@ private:
         }
     }
 
+public:
     // Per-thread dense subsampling workspace (the reference's countingGrid / averagingGrid / rgba,
     // octree_processor_impl.cu:41-71), reset cell by cell after each node.
     struct Workspace
@@ -599,6 +607,7 @@ private:
         std::vector<OutRecord> out;
     };
 
+private:
     // Gridding of an internal node (rp_subsampling.cu:16-60 / fp_subsampling.cu:15-58).
     Gridding nodeGridding(uint32_t level, const double nodeMin[3]) const
     {
@@ -614,11 +623,6 @@ private:
         const auto& c = itsB.cfg;
         auto& nodes   = itsB.nodes;
         const Node& node = nodes[nodeIdx];
-        Gridding gr      = nodeGridding(node.level, nodeMin);
-        const uint32_t gs = c.subsampling_grid;
-        const uint64_t cells = static_cast<uint64_t>(gs) * gs * gs;
-        const ColorMode mode = itsB.colorMode;
-        const uint32_t salt  = nodeSalt(c.seed, node.denseKey);
 
         // Gather the child points in slot order (the reference's blockIdx.z = slot).
         std::vector<Input> in;
@@ -651,6 +655,22 @@ private:
                 in.push_back(p);
             }
         }
+        subsampleInputs(nodeIdx, nodeMin, in, ws, subs[nodeIdx]);
+        nodes[nodeIdx].pointCount = static_cast<uint32_t>(subs[nodeIdx].lut.size());
+    }
+
+public:
+    // One node's subsample from the given inputs (child points in slot order with their colours).
+    void subsampleInputs(uint32_t nodeIdx, const double nodeMin[3], const std::vector<Input>& in, Workspace& ws,
+                         Subsample& sub)
+    {
+        const auto& c = itsB.cfg;
+        const Node& node  = itsB.nodes[nodeIdx];
+        Gridding gr       = nodeGridding(node.level, nodeMin);
+        const uint32_t gs = c.subsampling_grid;
+        const uint64_t cells = static_cast<uint64_t>(gs) * gs * gs;
+        const ColorMode mode = itsB.colorMode;
+        const uint32_t salt  = nodeSalt(c.seed, node.denseKey);
 
         // Evaluate: occupancy + winner (+ intra colour sums). rp_subsample_evaluation.cuh:37-203,
         // fp_subsample_evaluation.cuh:22-110, canonical winners A7/A8.
@@ -721,7 +741,6 @@ private:
         // Select: one point per occupied cell, slot order = ascending cell (A9). The record keeps
         // the winner's global quantised coordinates (rp_subsampling.cuh:116-121).
         std::sort(ws.touched.begin(), ws.touched.end());
-        Subsample& sub = subs[nodeIdx];
         sub.lut.resize(ws.touched.size());
         sub.out.resize(ws.touched.size());
         double sx = 1.0 / c.scale[0], sy = 1.0 / c.scale[1], sz = 1.0 / c.scale[2];
@@ -764,8 +783,20 @@ private:
                 std::fill_n(ws.rgbw.data() + 4 * static_cast<uint64_t>(cell), 4, 0.f);
         }
         ws.touched.clear();
-        nodes[nodeIdx].pointCount = static_cast<uint32_t>(sub.lut.size());
     }
+
+    void allocateWorkspace(Workspace& w) const
+    {
+        const uint64_t g     = itsB.cfg.subsampling_grid;
+        const uint64_t cells = g * g * g;
+        w.win.assign(cells, ~0ull);
+        if (itsB.colorMode == COLOR_INTRA || itsB.colorMode == COLOR_INTER)
+            w.color.assign(cells, 0);
+        if (itsB.colorMode == COLOR_INTER_WEIGHTED)
+            w.rgbw.assign(4 * cells, 0.f);
+    }
+
+private:
 
     // calculateChildMinBB (tools.cu:29-61), accumulated top-down exactly as the reference's DFS.
     void computeNodeMins(std::vector<double>& mins)
@@ -819,8 +850,7 @@ private:
     int subsample()
     {
         auto& nodes = itsB.nodes;
-        const auto& c = itsB.cfg;
-        std::vector<double> mins;
+        std::vector<double>& mins = itsB.nodeMins;
         computeNodeMins(mins);
 
         // statistics before subsampling (octree.cu:79-132): maxPointsPerNode sizes the reference's
@@ -832,7 +862,6 @@ private:
         itsB.maxLeafPoints = maxLeaf;
 
         std::vector<Subsample> subs(nodes.size());
-        const uint64_t cells = static_cast<uint64_t>(c.subsampling_grid) * c.subsampling_grid * c.subsampling_grid;
         std::vector<Workspace> ws(itsThreads);
         for (uint32_t l = 1; l <= itsB.lt.depth; ++l)
         {
@@ -848,13 +877,7 @@ private:
             parallelFor(nt, nt, [&](uint64_t, uint64_t, unsigned t) {
                 Workspace& w = ws[t];
                 if (w.win.empty())
-                {
-                    w.win.assign(cells, ~0ull);
-                    if (itsB.colorMode == COLOR_INTRA || itsB.colorMode == COLOR_INTER)
-                        w.color.assign(cells, 0);
-                    if (itsB.colorMode == COLOR_INTER_WEIGHTED)
-                        w.rgbw.assign(4 * cells, 0.f);
-                }
+                    allocateWorkspace(w);
                 for (;;)
                 {
                     uint64_t i = next.fetch_add(1);
@@ -1016,6 +1039,39 @@ void oracle_get_summary(void* h, uint64_t* out5)
     out5[4] = b->maxLeafPoints;
 }
 
+// Recomputes the subsample of internal node `nodeId` of a finished build from caller-supplied
+// inputs (point index + colour of every child point, children in slot order). Used to check the
+// distance-weighted colours level by level with the GPU's own child colours. Returns the number
+// of cells; outIdx / outRgb (3 per cell) receive the winners in ascending cell order.
+uint32_t oracle_subsample_node(void* h, const uint8_t* cloud, uint32_t nodeId, uint32_t n, const uint32_t* idx,
+                               const uint16_t* rgb, uint32_t* outIdx, uint16_t* outRgb)
+{
+    auto* b = static_cast<Build*>(h);
+    Builder builder(cloud, b->cfg, *b, true);
+    std::vector<Builder::Input> in(n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        in[i].pointIdx = idx[i];
+        loadPoint(cloud + static_cast<uint64_t>(idx[i]) * b->cfg.data_stride, b->cfg.cloud_type, in[i].x, in[i].y,
+                  in[i].z);
+        in[i].r = rgb[3 * i];
+        in[i].g = rgb[3 * i + 1];
+        in[i].b = rgb[3 * i + 2];
+    }
+    Builder::Workspace ws;
+    builder.allocateWorkspace(ws);
+    Builder::Subsample sub;
+    builder.subsampleInputs(nodeId, &b->nodeMins[3 * nodeId], in, ws, sub);
+    for (size_t k = 0; k < sub.lut.size(); ++k)
+    {
+        outIdx[k]         = sub.lut[k];
+        outRgb[3 * k]     = sub.out[k].r;
+        outRgb[3 * k + 1] = sub.out[k].g;
+        outRgb[3 * k + 2] = sub.out[k].b;
+    }
+    return static_cast<uint32_t>(sub.lut.size());
+}
+
 // ---- unit-level helpers -------------------------------------------------------------------------
 
 uint32_t oracle_map_point_to_grid(double x, double y, double z, uint32_t gridSize, double bbSize, double minX,
@@ -1044,11 +1100,14 @@ uint32_t oracle_random_rank(uint64_t seed, uint32_t nodeKey, uint32_t pointIdx)
     return randomRank(nodeSalt(seed, nodeKey), pointIdx);
 }
 
-// depth, then (depth + 1) entries of grid sizes, node amounts and dense offsets; returns D
+// depth, then (depth + 1) <= 10 entries of grid sizes, node amounts and dense offsets; returns D
+// (nothing is written for an unsupported grid, whose depth is 0)
 uint32_t oracle_level_tables(uint32_t grid, uint32_t* depth, uint32_t* grids, uint32_t* nodes, uint32_t* offsets)
 {
     LevelTables t = levelTables(grid);
     *depth        = t.depth;
+    if (t.depth == 0)
+        return 0;
     for (size_t l = 0; l < t.grid.size(); ++l)
     {
         grids[l]   = t.grid[l];

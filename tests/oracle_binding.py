@@ -63,6 +63,8 @@ def lib():
         L.oracle_hierarchy_size.restype = ctypes.c_uint64
         L.oracle_get_hierarchy.argtypes = [vp, vp]
         L.oracle_get_summary.argtypes = [vp, vp]
+        L.oracle_subsample_node.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp]
+        L.oracle_subsample_node.restype = ctypes.c_uint32
         L.oracle_map_point_to_grid.argtypes = [ctypes.c_double] * 3 + [ctypes.c_uint32] + [ctypes.c_double] * 4
         L.oracle_map_point_to_grid.restype = ctypes.c_uint32
         L.oracle_encode_colors.argtypes = [ctypes.c_uint16] * 3
@@ -95,6 +97,7 @@ class OracleResult:
     capacity: int = 0
     used: int = 0
     summary: np.ndarray = None    # exportedNodes, exportedPoints, internalNodes, referenceBoundBinds, maxLeaf
+    handle: object = None         # Handle when built with keep=True
 
     @property
     def finished(self):
@@ -128,7 +131,34 @@ def make_config(n, stride, cloud_type, cubic, scale, build, threads=0) -> Oracle
     return c
 
 
-def build(cloud: np.ndarray, n, stride, cloud_type, cubic, scale, spec, threads=0, want_data=True) -> OracleResult:
+class Handle:
+    """A live oracle build (for per-node recomputation); keeps the cloud buffer alive."""
+
+    def __init__(self, handle, cloud):
+        self.handle = handle
+        self.cloud = cloud
+
+    def subsample_node(self, node_id, idx, rgb):
+        n = len(idx)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        rgb = np.ascontiguousarray(rgb, np.uint16).reshape(-1)
+        out_idx = np.empty(max(n, 1), np.uint32)
+        out_rgb = np.empty(3 * max(n, 1), np.uint16)
+        k = lib().oracle_subsample_node(self.handle, self.cloud.ctypes.data, node_id, n, idx.ctypes.data,
+                                        rgb.ctypes.data, out_idx.ctypes.data, out_rgb.ctypes.data)
+        return out_idx[:k], out_rgb[:3 * k].reshape(k, 3)
+
+    def close(self):
+        if self.handle:
+            lib().oracle_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+def build(cloud: np.ndarray, n, stride, cloud_type, cubic, scale, spec, threads=0, want_data=True,
+          keep=False) -> OracleResult:
     L = lib()
     cfg = make_config(n, stride, cloud_type, cubic, scale, spec, threads)
     status = ctypes.c_int(0)
@@ -161,9 +191,13 @@ def build(cloud: np.ndarray, n, stride, cloud_type, cubic, scale, spec, threads=
             hb = np.empty(hs, np.uint8)
             L.oracle_get_hierarchy(h, hb.ctypes.data)
             r.hierarchy = hb.tobytes()
+        if keep:
+            r.handle = Handle(h, cloud)
+            h = None
         return r
     finally:
-        L.oracle_free(h)
+        if h:
+            L.oracle_free(h)
 
 
 def level_tables(grid):
