@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the full octree build on one MI355X.
+
+One step = one ocpi_generate_octree of the whole cloud (counting, merging, distribution,
+subsampling with colour filtering, final placement) with the cloud already resident in HBM
+(uploaded once by ocpi_init_octree, outside the timed region). The default workload is the
+morrowbay-sized cloud: 119,701,547 points, 27-byte records (3 x float64 + RGB, coordinates offset
+by 1e6), grids 512/128, merge threshold 10000, implicit subsampling with distance-weighted
+inter-cell colour filtering, output factor 2.2.
+
+Prints one JSON line. `roofline` is for the dominant kernel class, timed with HIP events on the
+session stream by the library; `cpu_baseline` times the CPU oracle on a bounded prefix of the same
+cloud on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "octree-cuda_amd"))
+
+import numpy as np  # noqa: E402
+
+from octree_amd import api  # noqa: E402
+from octree_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kind, n, stride, grid, summ):
+    """Compulsory DRAM bytes per launch of each kernel class (one input point = one unit)."""
+    internal_points = summ["used"] - n
+    if kind == "count":  # read every record, write the dense histogram
+        return n * stride + 4 * grid ** 3
+    if kind == "distribute":  # record, cell -> leaf lookup, index write, 18-byte record write
+        return n * (stride + 4 + 4 + 18)
+    return None
+
+
+def cpu_baseline(cloud, wl, cubic, sample_points, threads):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_binding as O
+
+    s = wl.cloud
+    sample = cloud[: sample_points * s.stride]
+    t0 = time.perf_counter()
+    r = O.build(sample, sample_points, s.stride, s.cloud_type, cubic, [s.scale] * 3, wl.build, threads=threads,
+                want_data=False)
+    dt = time.perf_counter() - t0
+    if r.status != 0:
+        return None
+    return {"value": sample_points / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (std::thread x {threads}) on the first {sample_points:,} points of the same "
+                      f"cloud with the same bounding box and configuration, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="C3", choices=sorted(W.WORKLOADS))
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000, help="points for the CPU baseline (0 = skip)")
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the octree build of one cloud "
+              f"runs on a single MI355X, see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    wl = W.WORKLOADS[args.workload]
+    s = wl.cloud
+    log(f"bench: generating {wl.name} ({s.points:,} points)")
+    t0 = time.perf_counter()
+    cloud = W.generate(s)
+    _, cubic = W.bounding_box(cloud, s.points, s.stride, s.cloud_type)
+    log(f"bench: cloud ready in {time.perf_counter() - t0:.1f} s, cubic bb {cubic}")
+
+    sess = api.Session(args.device)
+    sess.set_cloud(cloud, s.points, s.stride, s.cloud_type, cubic, [s.scale] * 3)
+    sess.configure(wl.build)
+    t0 = time.perf_counter()
+    sess.init()
+    log(f"bench: init (allocation + upload) {time.perf_counter() - t0:.2f} s")
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        sess.generate()
+        log(f"bench: warmup {i} {1e3 * (time.perf_counter() - t0):.2f} ms")
+
+    step_ms, timings = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        sess.generate()  # synchronous: returns after the stream has drained
+        step_ms.append(1e3 * (time.perf_counter() - t0))
+        timings.append(sess.timings())
+        log(f"bench: step {i} {step_ms[-1]:.2f} ms {timings[-1]}")
+    summ = sess.summary()
+    ms = float(np.mean(step_ms))
+    avg = {k: float(np.mean([t[k] for t in timings])) for k in timings[0]}
+
+    # dominant kernel class and its roofline
+    classes = {"count": avg["count_ms"], "distribute": avg["distribute_ms"],
+               "subsample_insert": avg["insert_ms"], "subsample_emit": avg["emit_ms"]}
+    dominant = max(classes, key=classes.get)
+    rl_kind = dominant if dominant in ("count", "distribute") else max(("count", "distribute"), key=classes.get)
+    rl_bytes = algorithmic_bytes(rl_kind, s.points, s.stride, wl.build.chunking_grid, summ)
+    rl_ms = classes[rl_kind]
+    achieved = rl_bytes / (rl_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": rl_kind, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": rl_bytes, "avg_launch_ms": round(rl_ms, 4),
+                "dominant_kernel_class": dominant,
+                "kernel_class_ms": {k: round(v, 4) for k, v in classes.items()}}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"bench: CPU oracle baseline on {args.cpu_sample:,} points with {threads} threads")
+        cpu = cpu_baseline(cloud, wl, cubic, min(args.cpu_sample, s.points), threads)
+
+    line = {
+        "metric": "points/sec full octree build (1 MI355X, % HBM roofline)",
+        "value": s.points / (ms * 1e-3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if s.cloud_type == 1 else "f32",
+        "data": "synthetic (seeded heightfield + Gaussian clusters, generated on the host)",
+        "config": {"workload": wl.name, "points": s.points, "record_bytes": s.stride,
+                   "chunking_grid": wl.build.chunking_grid, "subsampling_grid": wl.build.subsampling_grid,
+                   "merging_threshold": wl.build.merging_threshold, "output_factor": wl.build.output_factor,
+                   "intra": wl.build.intra, "inter": wl.build.inter, "weighted": wl.build.weighted,
+                   "explicit_random": wl.build.random, "sparse_nodes": summ["sparse_nodes"],
+                   "output_records": summ["used"], "stage_ms": {k: round(v, 4) for k, v in avg.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    sess.close()
+
+
+if __name__ == "__main__":
+    main()

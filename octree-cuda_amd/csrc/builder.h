@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -163,10 +164,12 @@ private:
     // output region [0, E): leaf points, then staged internal subsamples
     DeviceBuffer itsLut, itsOut, itsPts, itsStageNode, itsTmpLut, itsTmpOut;
 
-    // per-level subsampling workspace
-    DeviceBuffer itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid, itsNodeCap, itsTableBase, itsTileList;
-    DeviceBuffer itsKeys, itsWin, itsColor, itsCellCnt, itsCellStart, itsCellFill, itsSlot;
-    DeviceBuffer itsEntryOf, itsPosOf, itsCsr;
+    // per-level subsampling workspace: internal node list, input segments and tiles, per-node
+    // occupancy bitmaps with their popcount prefix, per-cell (dense index) state, per-input state
+    DeviceBuffer itsInternalList, itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid, itsTileList;
+    DeviceBuffer itsBitmap, itsPrefix, itsNodeFirst;
+    DeviceBuffer itsWin, itsColor, itsCellCnt, itsCellStart, itsCellKey;
+    DeviceBuffer itsCellOf, itsCompactOf, itsRankOf, itsPayload;
 
     // small device scalars and scan scratch
     DeviceBuffer itsScalars, itsScanScratch;
@@ -182,9 +185,11 @@ private:
         EV_END,
         EV_SLOTS
     };
-    hipEvent_t itsEvents[EV_SLOTS]              = {};
-    hipEvent_t itsLevelEvents[MAX_LEVELS][4]    = {};  // insert begin/end, emit begin/end per level
-    uint32_t itsLevelsRun                       = 0;
+    hipEvent_t itsEvents[EV_SLOTS] = {};
+    // per subsampling batch: evaluate begin/end, emit begin/end
+    std::vector<std::array<hipEvent_t, 4>> itsBatchEvents;
+    uint32_t itsBatchesRun = 0;
+    std::array<hipEvent_t, 4>& batchEvents();
 };
 
 }  // namespace oc

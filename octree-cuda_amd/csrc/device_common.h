@@ -94,59 +94,6 @@ __host__ __device__ __forceinline__ uint32_t nodeSalt(uint64_t seed, uint32_t no
             splitmix64Mix(seed + 0x9E3779B97F4A7C15ULL * (static_cast<uint64_t>(nodeKey) + 1)) >> 32);
 }
 
-// ---- per-node open-addressing cell tables ----------------------------------------------------------
-//
-// Each internal node of a level owns a contiguous slice [base, base + cap) of the level's table
-// with cap > 2 * (its input points), so a probe always reaches an empty slot and every loop
-// below terminates. Keys are subsampling-cell indices (< 2^30), EMPTY_KEY marks a free slot.
-
-constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
-
-__device__ __forceinline__ uint32_t tableHome(uint32_t cell, uint32_t cap)
-{
-    return static_cast<uint32_t>((static_cast<uint64_t>(fmix32(cell)) * cap) >> 32);
-}
-
-// Returns the slot of `cell`, inserting it if absent; *inserted tells whether this call claimed it.
-__device__ __forceinline__ uint32_t tableInsert(uint32_t* keys, uint32_t cap, uint32_t cell, bool& inserted)
-{
-    uint32_t i = tableHome(cell, cap);
-    inserted   = false;
-    for (;;)
-    {
-        uint32_t k = __hip_atomic_load(keys + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == cell)
-            return i;
-        if (k == EMPTY_KEY)
-        {
-            uint32_t prev = atomicCAS(keys + i, EMPTY_KEY, cell);
-            if (prev == EMPTY_KEY)
-            {
-                inserted = true;
-                return i;
-            }
-            if (prev == cell)
-                return i;
-        }
-        i = (i + 1 == cap) ? 0 : i + 1;
-    }
-}
-
-// Returns the slot of `cell` or EMPTY_KEY when absent (keys are stable while this is called).
-__device__ __forceinline__ uint32_t tableFind(const uint32_t* keys, uint32_t cap, uint32_t cell)
-{
-    uint32_t i = tableHome(cell, cap);
-    for (;;)
-    {
-        uint32_t k = keys[i];
-        if (k == cell)
-            return i;
-        if (k == EMPTY_KEY)
-            return EMPTY_KEY;
-        i = (i + 1 == cap) ? 0 : i + 1;
-    }
-}
-
 __device__ __forceinline__ void storeRecord(OutRecord* dst, const OutRecord& r)
 {
     __builtin_memcpy(dst, &r, sizeof(OutRecord));

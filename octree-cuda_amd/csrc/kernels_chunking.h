@@ -1,0 +1,301 @@
+// Kernels of the counting, merging and distribution phases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.h"
+#include "types.h"
+
+namespace oc {
+
+constexpr int BLOCK    = 256;
+
+// scalar slots (u64) in itsScalars
+enum ScalarSlot : int
+{
+    SC_ERROR       = 0,
+    SC_M           = 1,
+    SC_LEAF_TOTAL  = 2,
+    SC_SEG_TOTAL   = 3,  // packed (tiles << 32 | inputs)
+    SC_TABLE_TOTAL = 4,
+    SC_LEVEL_START = 8,  // u32[MAX_LEVELS + 1] packed into the following slots
+    SC_COUNT       = 16
+};
+
+constexpr uint32_t ERR_OUTSIDE  = 1u;
+constexpr uint32_t ERR_CAPACITY = 2u;
+
+inline unsigned gridFor(uint64_t n, uint64_t cap = 65536)
+{
+    uint64_t b = (n + BLOCK - 1) / BLOCK;
+    if (b == 0)
+        b = 1;
+    return static_cast<unsigned>(b < cap ? b : cap);
+}
+
+__global__ __launch_bounds__(BLOCK) void kFill32(uint32_t* p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(BLOCK) void kFill64(uint64_t* p, uint64_t n, uint64_t v)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+        p[i] = v;
+}
+
+// K1: kernelPointCounting (point_counting.cuh:30-53) - dense level-0 histogram. Sparse ids are
+// assigned afterwards by a scan (canonical order), so no global id counter is touched here.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kCount(const uint8_t* __restrict__ cloud, uint32_t n, uint32_t stride,
+                                                 Gridding g, uint32_t* __restrict__ count, uint32_t* __restrict__ err)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        double x, y, z;
+        loadXYZ<T>(cloud + i * stride, x, y, z);
+        uint64_t ix, iy, iz;
+        gridCoords(x, y, z, g, ix, iy, iz);
+        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
+        {
+            atomicOr(err, ERR_OUTSIDE);
+            continue;
+        }
+        atomicAdd(count + static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize), 1u);
+    }
+}
+
+// K2 (counts only): kernelPropagatePointCounts (point_count_propagation.cuh:34-86). Child slot k
+// of parent (x, y, z) is (2x + (k>>2 & 1), 2y + (k>>1 & 1), 2z + (k & 1)).
+__global__ __launch_bounds__(BLOCK) void kPropagate(uint32_t* __restrict__ count, uint32_t offset, uint32_t grid,
+                                                     uint32_t childOffset, uint32_t childGrid)
+{
+    const uint64_t n  = static_cast<uint64_t>(grid) * grid * grid;
+    const uint32_t cxy = childGrid * childGrid;
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        uint32_t idx = static_cast<uint32_t>(i);
+        uint32_t xy  = grid * grid;
+        uint32_t z = idx / xy, y = (idx - z * xy) / grid, x = (idx - z * xy) % grid;
+        const uint32_t* c = count + childOffset + 2 * z * cxy + 2 * y * childGrid + 2 * x;
+        uint32_t sum      = c[0] + c[1] + c[childGrid] + c[childGrid + 1] + c[cxy] + c[cxy + 1] + c[cxy + childGrid] +
+                       c[cxy + childGrid + 1];
+        count[offset + idx] = sum;
+    }
+}
+
+struct OccupiedIn
+{
+    const uint32_t* count;
+    __device__ uint32_t operator()(uint64_t d) const { return count[d] ? 1u : 0u; }
+};
+
+// Sparse id = exclusive rank of the occupied dense cell (A2); the first id of every level is
+// recorded, and the node's key and dense count are written in the same pass.
+struct OccupiedOut
+{
+    int32_t* denseToSparse;
+    uint32_t* nodeKey;
+    uint32_t* nodeCount;
+    const uint32_t* count;
+    uint32_t* levelStart;
+    uint32_t offsets[MAX_LEVELS];
+    uint32_t levels;
+    __device__ void operator()(uint64_t d, uint32_t prefix, uint32_t v) const
+    {
+        denseToSparse[d] = v ? static_cast<int32_t>(prefix) : -1;
+        if (v)
+        {
+            nodeKey[prefix]   = static_cast<uint32_t>(d);
+            nodeCount[prefix] = count[d];
+        }
+        for (uint32_t l = 0; l < levels; ++l)
+            if (d == offsets[l])
+                levelStart[l] = prefix;
+    }
+};
+
+struct LevelParams
+{
+    uint32_t offset[MAX_LEVELS];
+    uint32_t grid[MAX_LEVELS];
+    uint32_t depth;
+    uint32_t threshold;
+};
+
+__device__ __forceinline__ uint32_t levelOfKey(uint32_t key, const LevelParams& lp)
+{
+    uint32_t l = 0;
+    for (uint32_t k = 1; k <= lp.depth; ++k)
+        if (key >= lp.offset[k])
+            l = k;
+    return l;
+}
+
+// Node links and flags: kernelInitLeafNodes (octree_initialization.cuh:24-52) and the structural
+// part of kernelMergeHierarchical (hierarchical_merging.cuh:35-122):
+//   internal  = level >= 1 && count >= threshold
+//   finished  = parent is internal (the root is always finished, A13)
+//   pointCount = internal ? 0 : count
+// and the size of the leaf data block an internal node hands out to its leaf children.
+__global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
+                                                     const uint32_t* __restrict__ nodeCount,
+                                                     const int32_t* __restrict__ denseToSparse,
+                                                     uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
+                                                     int32_t* __restrict__ parent,
+                                                     int32_t* __restrict__ children, uint8_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ fill, uint32_t* __restrict__ blockSize,
+                                                     uint32_t* __restrict__ subtree)
+{
+    for (uint64_t i = globalThread(); i < m; i += globalStride())
+    {
+        uint32_t id  = static_cast<uint32_t>(i);
+        uint32_t key = nodeKey[id];
+        uint32_t l   = levelOfKey(key, lp);
+        uint32_t g   = lp.grid[l];
+        uint32_t lin = key - lp.offset[l];
+        uint32_t x = lin % g, y = (lin / g) % g, z = lin / (g * g);
+        uint32_t cnt  = nodeCount[id];
+        bool internal = l >= 1 && cnt >= lp.threshold;
+
+        int32_t p = -1;
+        if (l < lp.depth)
+        {
+            uint32_t pg = lp.grid[l + 1];
+            p           = denseToSparse[lp.offset[l + 1] + (x >> 1) + (y >> 1) * pg + (z >> 1) * pg * pg];
+        }
+        bool finished = (p < 0) ? true : (nodeCount[p] >= lp.threshold);
+
+        uint32_t blk = 0;
+        if (l >= 1)
+        {
+            uint32_t cg = lp.grid[l - 1];
+            for (int k = 0; k < 8; ++k)
+            {
+                uint32_t cx = 2 * x + ((k >> 2) & 1), cy = 2 * y + ((k >> 1) & 1), cz = 2 * z + (k & 1);
+                int32_t c   = denseToSparse[lp.offset[l - 1] + cx + cy * cg + cz * cg * cg];
+                children[8 * static_cast<uint64_t>(id) + k] = c;
+                if (internal && c >= 0)
+                {
+                    uint32_t cc        = nodeCount[c];
+                    bool childInternal = (l - 1) >= 1 && cc >= lp.threshold;
+                    if (!childInternal)
+                        blk += cc;
+                }
+            }
+        }
+        else
+        {
+            for (int k = 0; k < 8; ++k)
+                children[8 * static_cast<uint64_t>(id) + k] = -1;
+        }
+        pointCount[id] = internal ? 0u : cnt;
+        dataIdx[id]    = 0;
+        parent[id]     = p;
+        flags[id]      = static_cast<uint8_t>((finished ? NODE_FINISHED : 0) | (internal ? NODE_INTERNAL : 0));
+        fill[id]       = 0;
+        blockSize[id]  = blk;
+        subtree[id]    = 0;
+    }
+}
+
+struct ArrayIn
+{
+    const uint32_t* v;
+    __device__ uint32_t operator()(uint64_t i) const { return v[i]; }
+};
+
+struct ExclusiveToArray
+{
+    uint32_t* dst;
+    __device__ void operator()(uint64_t i, uint32_t prefix, uint32_t) const { dst[i] = prefix; }
+};
+
+// Leaf data offsets (A4): an internal node's block starts at the exclusive scan of block sizes in
+// sparse-id order (= parent level, parent linear index); children follow in slot order.
+__global__ __launch_bounds__(BLOCK) void kLeafDataIdx(uint32_t m, uint32_t threshold,
+                                                       const uint32_t* __restrict__ blockStart,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const int32_t* __restrict__ children,
+                                                       const uint32_t* __restrict__ pointCount,
+                                                       uint64_t* __restrict__ dataIdx)
+{
+    for (uint64_t i = globalThread(); i < m; i += globalStride())
+    {
+        uint32_t id = static_cast<uint32_t>(i);
+        if (!(flags[id] & NODE_INTERNAL))
+        {
+            if (id == m - 1)
+                dataIdx[id] = 0;  // root as the only leaf (A13)
+            continue;
+        }
+        uint64_t r = blockStart[id];
+        for (int k = 0; k < 8; ++k)
+        {
+            int32_t c = children[8 * static_cast<uint64_t>(id) + k];
+            if (c >= 0 && !(flags[c] & NODE_INTERNAL))
+            {
+                dataIdx[c] = r;
+                r += pointCount[c];
+            }
+        }
+    }
+}
+
+// Level-0 cell -> its leaf (the first finished ancestor-or-self, point_distributing.cuh:56-60),
+// written over the level-0 counts so the distribution needs one lookup per point.
+__global__ __launch_bounds__(BLOCK) void kCellLeaf(uint32_t level0Nodes, const uint32_t* __restrict__ nodeKey,
+                                                    const uint8_t* __restrict__ flags,
+                                                    const int32_t* __restrict__ parent, uint32_t* __restrict__ count)
+{
+    for (uint64_t i = globalThread(); i < level0Nodes; i += globalStride())
+    {
+        int32_t s = static_cast<int32_t>(i);
+        while (!(flags[s] & NODE_FINISHED))
+            s = parent[s];
+        count[nodeKey[i]] = static_cast<uint32_t>(s);
+    }
+}
+
+// K5: kernelDistributePoints (point_distributing.cuh:31-76). Slot within the leaf by atomic
+// (order unconstrained, A5); writes the point index, the quantised record and the coordinates.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kDistribute(const uint8_t* __restrict__ cloud, uint32_t n, uint32_t stride,
+                                                      Gridding g, const uint32_t* __restrict__ cellLeaf,
+                                                      uint32_t* __restrict__ fill,
+                                                      const uint64_t* __restrict__ dataIdx,
+                                                      uint32_t* __restrict__ lut, OutRecord* __restrict__ out,
+                                                      T* __restrict__ pts, double sx, double sy, double sz)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        const uint8_t* rec = cloud + i * stride;
+        T v[3];
+        __builtin_memcpy(v, rec, sizeof(v));
+        double x = v[0], y = v[1], z = v[2];
+        uint64_t ix, iy, iz;
+        gridCoords(x, y, z, g, ix, iy, iz);
+        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
+            continue;  // reported by the counting pass
+        uint32_t cell = static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
+        uint32_t leaf = cellLeaf[cell];
+        uint32_t slot = atomicAdd(fill + leaf, 1u);
+        uint32_t pos  = static_cast<uint32_t>(dataIdx[leaf]) + slot;
+        lut[pos]      = static_cast<uint32_t>(i);
+        OutRecord o;
+        o.x = static_cast<int32_t>(floor((x - g.minX) * sx));
+        o.y = static_cast<int32_t>(floor((y - g.minY) * sy));
+        o.z = static_cast<int32_t>(floor((z - g.minZ) * sz));
+        const uint8_t* col = rec + 3 * sizeof(T);
+        o.r = col[0];
+        o.g = col[1];
+        o.b = col[2];
+        storeRecord(out + pos, o);
+        __builtin_memcpy(pts + 3 * static_cast<uint64_t>(pos), v, sizeof(v));
+    }
+}
+
+}  // namespace oc

@@ -1,0 +1,529 @@
+// Kernels of the level-batched subsampling and the final DFS post-order placement.
+//
+// All internal nodes of one octree level are processed together (in memory-bounded batches of
+// nodes). Each node of a batch owns an occupancy bitmap over its g^3 subsampling grid and the
+// exclusive popcount prefix of that bitmap's words, i.e. a rank dictionary: the occupied cells of
+// the batch get dense indices in (node, ascending cell) order. Per batch:
+//   kMark        input point -> cell (the reference's mapPointToGrid with the node's gridding),
+//                set the cell's bit
+//   prefix scan  popcounts of all bitmap words -> dense index of every occupied cell
+//   kAccumulate  winner per cell (atomicMin of rank<<32 | input position), packed colour sums,
+//                per-cell point counts for the weighted filter
+//   kScatterPayload (weighted only) point coordinates + colour in cell order
+//   kEmit        one output record per occupied cell, in ascending cell order per node
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.h"
+#include "kernels_chunking.h"
+#include "types.h"
+
+namespace oc {
+
+constexpr uint32_t TILE = 256;  // inputs per workgroup in the per-input kernels
+
+struct SubsampleParams
+{
+    double cubicMin[3];
+    double sideHalf[MAX_LEVELS];  // side of level L / 2 (the child offset when descending from L)
+    double side;                  // side of the current level
+    uint32_t level;
+    uint32_t depth;
+    uint32_t levelOffset;
+    uint32_t levelGrid;
+    uint32_t g;  // subsampling grid
+    uint64_t seed;
+};
+
+// internal nodes of a level -> compact list
+struct InternalIn
+{
+    const uint8_t* flags;
+    uint32_t levelStart;
+    __device__ uint32_t operator()(uint64_t i) const { return (flags[levelStart + i] & NODE_INTERNAL) ? 1u : 0u; }
+};
+
+struct InternalOut
+{
+    uint32_t* list;
+    uint32_t levelStart;
+    __device__ void operator()(uint64_t i, uint32_t prefix, uint32_t v) const
+    {
+        if (v)
+            list[prefix] = levelStart + static_cast<uint32_t>(i);
+    }
+};
+
+// Per internal node k of the level: its eight input segments (children in slot order, leaf data
+// or the child's staged subsample) and its gridding. The node minimum is accumulated top-down
+// exactly like the reference's DFS (calculateChildMinBB, tools.cu:29-61; rp_subsampling.cu:16-60):
+// x = min.x; for every step down whose slot has the x bit, x = x + side/2 of that step; the grid
+// extent is bbSize = (min.x + side) - min.x.
+__global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp,
+                                                    const uint32_t* __restrict__ internalList,
+                                                    const uint32_t* __restrict__ nodeKey,
+                                                    const uint8_t* __restrict__ flags,
+                                                    const int32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ pointCount,
+                                                    const uint64_t* __restrict__ dataIdx,
+                                                    const uint32_t* __restrict__ stage,
+                                                    uint32_t* __restrict__ segCount, uint32_t* __restrict__ segSrc,
+                                                    NodeGrid* __restrict__ nodeGrid)
+{
+    for (uint64_t k = globalThread(); k < nodes; k += globalStride())
+    {
+        uint32_t id = internalList[k];
+        for (int s = 0; s < 8; ++s)
+        {
+            int32_t c    = children[8 * static_cast<uint64_t>(id) + s];
+            uint32_t cnt = 0, src = 0;
+            if (c >= 0)
+            {
+                cnt = pointCount[c];
+                src = (flags[c] & NODE_INTERNAL) ? stage[c] : static_cast<uint32_t>(dataIdx[c]);
+            }
+            segCount[8 * k + s] = cnt;
+            segSrc[8 * k + s]   = src;
+        }
+        uint32_t key = nodeKey[id];
+        uint32_t lin = key - sp.levelOffset;
+        uint32_t gl  = sp.levelGrid;
+        uint32_t x = lin % gl, y = (lin / gl) % gl, z = lin / (gl * gl);
+        double mx = sp.cubicMin[0], my = sp.cubicMin[1], mz = sp.cubicMin[2];
+        for (uint32_t L = sp.depth; L > sp.level; --L)
+        {
+            uint32_t b = L - 1 - sp.level;
+            if ((x >> b) & 1)
+                mx = mx + sp.sideHalf[L];
+            if ((y >> b) & 1)
+                my = my + sp.sideHalf[L];
+            if ((z >> b) & 1)
+                mz = mz + sp.sideHalf[L];
+        }
+        NodeGrid ng;
+        ng.grid.minX     = mx;
+        ng.grid.minY     = my;
+        ng.grid.minZ     = mz;
+        ng.grid.bbSize   = (mx + sp.side) - mx;
+        ng.grid.gridSize = sp.g;
+        double t         = ng.grid.bbSize / sp.g;
+        ng.diag          = sqrt((t * 3) * (t * 3) * 3) / 2.0;
+        ng.salt          = nodeSalt(sp.seed, key);
+        ng.id            = id;
+        nodeGrid[k]      = ng;
+    }
+}
+
+// packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums
+struct SegmentIn
+{
+    const uint32_t* segCount;
+    __device__ uint64_t operator()(uint64_t s) const
+    {
+        uint64_t c = segCount[s];
+        return (((c + TILE - 1) / TILE) << 32) | c;
+    }
+};
+
+struct SegmentOut
+{
+    uint32_t* segTile;
+    uint32_t* segInput;
+    __device__ void operator()(uint64_t s, uint64_t prefix, uint64_t) const
+    {
+        segTile[s]  = static_cast<uint32_t>(prefix >> 32);
+        segInput[s] = static_cast<uint32_t>(prefix);
+    }
+};
+
+__global__ __launch_bounds__(BLOCK) void kTileList(uint64_t segments, const uint32_t* __restrict__ segCount,
+                                                    const uint32_t* __restrict__ segTile, uint32_t* __restrict__ tileList)
+{
+    for (uint64_t s = globalThread(); s < segments; s += globalStride())
+    {
+        uint32_t tiles = (segCount[s] + TILE - 1) / TILE;
+        uint32_t first = segTile[s];
+        for (uint32_t t = 0; t < tiles; ++t)
+            tileList[first + t] = static_cast<uint32_t>(s);
+    }
+}
+
+// Identifies the input of this thread: tile -> segment -> (node k, input position, input index q).
+struct InputRef
+{
+    uint32_t seg, local, k, pos, q;
+    bool valid;
+};
+
+__device__ __forceinline__ InputRef inputOf(uint32_t tile, const uint32_t* tileList, const uint32_t* segTile,
+                                            const uint32_t* segCount, const uint32_t* segSrc,
+                                            const uint32_t* segInput)
+{
+    InputRef r;
+    r.seg   = tileList[tile];
+    r.local = (tile - segTile[r.seg]) * TILE + threadIdx.x;
+    r.valid = r.local < segCount[r.seg];
+    r.k     = r.seg >> 3;
+    r.pos   = segSrc[r.seg] + r.local;
+    r.q     = segInput[r.seg] + r.local;
+    return r;
+}
+
+// Evaluate, part 1 (rp_subsample_evaluation.cuh:37-203, fp_subsample_evaluation.cuh:22-110):
+// the cell of every input point in its node's grid; marks the cell occupied.
+template <typename T>
+__global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
+                                               const uint32_t* __restrict__ tileList,
+                                               const uint32_t* __restrict__ segTile,
+                                               const uint32_t* __restrict__ segCount,
+                                               const uint32_t* __restrict__ segSrc,
+                                               const uint32_t* __restrict__ segInput,
+                                               const NodeGrid* __restrict__ nodeGrid, const T* __restrict__ pts,
+                                               uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
+                                               uint32_t* __restrict__ err)
+{
+    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    if (!in.valid)
+        return;
+    const NodeGrid ng = nodeGrid[in.k];
+    const T* p        = pts + 3 * static_cast<uint64_t>(in.pos);
+    uint64_t ix, iy, iz;
+    gridCoords(static_cast<double>(p[0]), static_cast<double>(p[1]), static_cast<double>(p[2]), ng.grid, ix, iy, iz);
+    const uint32_t gs = ng.grid.gridSize;
+    if (ix >= gs || iy >= gs || iz >= gs)
+    {
+        atomicOr(err, ERR_OUTSIDE);
+        cellOf[in.q] = 0;
+        return;
+    }
+    const uint32_t cell = static_cast<uint32_t>(ix + iy * gs + iz * gs * gs);
+    atomicOr(bitmap + (in.k - nodeBase) * wordsPerNode + (cell >> 5), 1u << (cell & 31));
+    cellOf[in.q] = cell;
+}
+
+struct PopcountIn
+{
+    const uint32_t* bitmap;
+    __device__ uint32_t operator()(uint64_t w) const { return __popc(bitmap[w]); }
+};
+
+// dense index of a set cell: exclusive popcount prefix of its word + set bits below it
+__device__ __forceinline__ uint32_t cellRank(const uint32_t* bitmap, const uint32_t* prefix, uint64_t word,
+                                             uint32_t bit)
+{
+    return prefix[word] + __popc(bitmap[word] & ((1u << bit) - 1u));
+}
+
+// Per node of the batch: subsample size (occupied cells), staging position, first dense index.
+__global__ __launch_bounds__(BLOCK) void kBatchNodes(uint32_t nodeBase, uint32_t nodes, uint64_t wordsPerNode,
+                                                      const NodeGrid* __restrict__ nodeGrid,
+                                                      const uint32_t* __restrict__ prefix, uint32_t cursor,
+                                                      uint32_t* __restrict__ pointCount, uint32_t* __restrict__ stage,
+                                                      uint32_t* __restrict__ nodeFirst)
+{
+    for (uint64_t kl = globalThread(); kl <= nodes; kl += globalStride())
+    {
+        uint32_t b    = prefix[kl * wordsPerNode];
+        nodeFirst[kl] = b;
+        if (kl == nodes)
+            continue;
+        uint32_t e     = prefix[(kl + 1) * wordsPerNode];
+        uint32_t id    = nodeGrid[nodeBase + kl].id;
+        pointCount[id] = e - b;
+        stage[id]      = cursor + b;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __restrict__ win,
+                                                        uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        win[i] = ~0ull;
+        if (color)
+            color[i] = 0;
+        if (cellCnt)
+            cellCnt[i] = 0;
+    }
+}
+
+// Evaluate, part 2: winner (A7/A8: atomicMin of rank << 32 | input position, rank = point index
+// for the implicit method or the keyed bijective hash of it for the explicit one), the packed
+// colour sum of the input records (intra-cell, and the per-cell sums the inter-cell filter is
+// built from), and per-cell point counts + each point's rank in its cell (weighted filter).
+template <uint32_t MODE, bool RANDOM>
+__global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                     const uint32_t* __restrict__ tileList,
+                                                     const uint32_t* __restrict__ segTile,
+                                                     const uint32_t* __restrict__ segCount,
+                                                     const uint32_t* __restrict__ segSrc,
+                                                     const uint32_t* __restrict__ segInput,
+                                                     const NodeGrid* __restrict__ nodeGrid,
+                                                     const uint32_t* __restrict__ bitmap,
+                                                     const uint32_t* __restrict__ prefix,
+                                                     const uint32_t* __restrict__ cellOf,
+                                                     const uint32_t* __restrict__ lut,
+                                                     const OutRecord* __restrict__ out, uint64_t* __restrict__ win,
+                                                     uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
+                                                     uint32_t* __restrict__ cellKey, uint32_t* __restrict__ compactOf,
+                                                     uint32_t* __restrict__ rankOf)
+{
+    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    if (!in.valid)
+        return;
+    const uint32_t cell = cellOf[in.q];
+    const uint64_t word = (in.k - nodeBase) * wordsPerNode + (cell >> 5);
+    if (!((bitmap[word] >> (cell & 31)) & 1u))
+        return;  // the point lay outside its node's grid (reported by kMark)
+    const uint32_t c = cellRank(bitmap, prefix, word, cell & 31);
+    const uint32_t idx  = lut[in.pos];
+    const uint32_t rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
+    atomicMin(reinterpret_cast<unsigned long long*>(win + c),
+              static_cast<unsigned long long>((static_cast<uint64_t>(rank) << 32) | in.pos));
+    // every point of the cell stores the same key
+    __hip_atomic_store(cellKey + c, cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
+    {
+        OutRecord r = loadRecord(out + in.pos);
+        atomicAdd(reinterpret_cast<unsigned long long*>(color + c),
+                  static_cast<unsigned long long>(encodeColors(r.r, r.g, r.b)));
+    }
+    if (MODE == COLOR_INTER_WEIGHTED)
+    {
+        compactOf[in.q] = c;
+        rankOf[in.q]    = atomicAdd(cellCnt + c, 1u);
+    }
+}
+
+// Point payload of the weighted filter, stored in cell order so that a cell's points are one
+// contiguous run (coordinates in the cloud's type and the input record colour).
+template <typename T>
+struct alignas(sizeof(T) == 8 ? 16 : 4) Payload
+{
+    T x, y, z;
+    uint16_t r, g, b, pad;
+};
+
+template <typename T>
+__global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
+                                                         const uint32_t* __restrict__ segTile,
+                                                         const uint32_t* __restrict__ segCount,
+                                                         const uint32_t* __restrict__ segSrc,
+                                                         const uint32_t* __restrict__ segInput,
+                                                         const uint32_t* __restrict__ compactOf,
+                                                         const uint32_t* __restrict__ rankOf,
+                                                         const uint32_t* __restrict__ cellStart,
+                                                         const T* __restrict__ pts, const OutRecord* __restrict__ out,
+                                                         Payload<T>* __restrict__ payload)
+{
+    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    if (!in.valid)
+        return;
+    const T* p  = pts + 3 * static_cast<uint64_t>(in.pos);
+    OutRecord r = loadRecord(out + in.pos);
+    Payload<T> pl;
+    pl.x   = p[0];
+    pl.y   = p[1];
+    pl.z   = p[2];
+    pl.r   = r.r;
+    pl.g   = r.g;
+    pl.b   = r.b;
+    pl.pad = 0;
+    payload[cellStart[compactOf[in.q]] + rankOf[in.q]] = pl;
+}
+
+// node of dense index c: last kl with nodeFirst[kl] <= c (nodes without cells share the next
+// node's first index)
+__device__ __forceinline__ uint32_t nodeOfCell(const uint32_t* nodeFirst, uint32_t nodes, uint32_t c)
+{
+    uint32_t lo = 0, hi = nodes;
+    while (hi - lo > 1)
+    {
+        uint32_t mid = (lo + hi) >> 1;
+        if (nodeFirst[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Select (rp_subsampling.cuh:70-321, fp_subsampling.cuh:30-306): one output record per occupied
+// cell, written at cursor + its dense index (node by node, cells ascending). The record keeps the
+// winner's global quantised position; the colour is the winner's (no filter), the decoded intra
+// sum, the decoded sum of the per-cell sums over the 3x3x3 neighbourhood (inter; equal to the
+// reference's per-point scatter into occupied neighbours because integer sums commute), or the
+// gathered distance-weighted average (kernel_inter_cell_filtering.cuh:109-194) over the points
+// of the neighbourhood, with the weight of a point p for this cell c:
+//   res = float(sqrt(|centre(c) - p|^2) / diag);  w = expf(-(res * res) * 10).
+template <typename T, uint32_t MODE>
+__global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint32_t nodes,
+                                                uint64_t wordsPerNode, uint32_t cursor, uint32_t N,
+                                                const uint32_t* __restrict__ nodeFirst,
+                                                const NodeGrid* __restrict__ nodeGrid,
+                                                const uint32_t* __restrict__ bitmap,
+                                                const uint32_t* __restrict__ prefix,
+                                                const uint32_t* __restrict__ cellKey,
+                                                const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
+                                                const uint32_t* __restrict__ cellCnt,
+                                                const uint32_t* __restrict__ cellStart,
+                                                const Payload<T>* __restrict__ payload, uint32_t* __restrict__ lut,
+                                                OutRecord* __restrict__ out, T* __restrict__ pts,
+                                                uint32_t* __restrict__ stageNode)
+{
+    for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
+    {
+        const uint32_t c    = static_cast<uint32_t>(ci);
+        const uint32_t kl   = nodeOfCell(nodeFirst, nodes, c);
+        const NodeGrid ng   = nodeGrid[nodeBase + kl];
+        const uint32_t wpos = static_cast<uint32_t>(win[c] & 0xFFFFFFFFull);
+        OutRecord rec       = loadRecord(out + wpos);
+        const uint32_t dst  = cursor + c;
+
+        if (MODE == COLOR_INTRA)
+        {
+            decodeColors(color[c], rec.r, rec.g, rec.b);
+        }
+        else if (MODE == COLOR_INTER || MODE == COLOR_INTER_WEIGHTED)
+        {
+            const uint32_t key   = cellKey[c];
+            const uint32_t gs    = ng.grid.gridSize;
+            const uint64_t words = static_cast<uint64_t>(kl) * wordsPerNode;
+            const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>((key / gs) % gs),
+                          cz = static_cast<int32_t>(key / (gs * gs));
+            uint64_t sum   = 0;
+            float acc[4]   = {0.f, 0.f, 0.f, 0.f};
+            const double t = ng.grid.bbSize / gs;
+            const double ctrX = ng.grid.minX + cx * t + (t * 0.5);
+            const double ctrY = ng.grid.minY + cy * t + (t * 0.5);
+            const double ctrZ = ng.grid.minZ + cz * t + (t * 0.5);
+            for (int ox = -1; ox <= 1; ++ox)
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int oz = -1; oz <= 1; ++oz)
+                    {
+                        const int32_t nx = cx + ox, ny = cy + oy, nz = cz + oz;
+                        if (nx < 0 || ny < 0 || nz < 0 || nx >= static_cast<int32_t>(gs) ||
+                            ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
+                            continue;
+                        const uint32_t ncell = static_cast<uint32_t>(nx) + static_cast<uint32_t>(ny) * gs +
+                                               static_cast<uint32_t>(nz) * gs * gs;
+                        const uint64_t w  = words + (ncell >> 5);
+                        const uint32_t bw = bitmap[w];
+                        if (!((bw >> (ncell & 31)) & 1u))
+                            continue;
+                        const uint32_t nc = prefix[w] + __popc(bw & ((1u << (ncell & 31)) - 1u));
+                        if (MODE == COLOR_INTER)
+                        {
+                            sum += color[nc];
+                        }
+                        else
+                        {
+                            const uint32_t first = cellStart[nc];
+                            const uint32_t cnt   = cellCnt[nc];
+                            for (uint32_t q = 0; q < cnt; ++q)
+                            {
+                                const Payload<T> pl = payload[first + q];
+                                const double uX     = ctrX - static_cast<double>(pl.x);
+                                const double uY     = ctrY - static_cast<double>(pl.y);
+                                const double uZ     = ctrZ - static_cast<double>(pl.z);
+                                float res = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
+                                res       = expf(-((res * res) * 10.f));
+                                acc[0] += static_cast<float>(pl.r) * res;
+                                acc[1] += static_cast<float>(pl.g) * res;
+                                acc[2] += static_cast<float>(pl.b) * res;
+                                acc[3] += res;
+                            }
+                        }
+                    }
+            if (MODE == COLOR_INTER)
+            {
+                decodeColors(sum, rec.r, rec.g, rec.b);
+            }
+            else
+            {
+                rec.r = static_cast<uint16_t>(acc[0] / acc[3]);
+                rec.g = static_cast<uint16_t>(acc[1] / acc[3]);
+                rec.b = static_cast<uint16_t>(acc[2] / acc[3]);
+            }
+        }
+        storeRecord(out + dst, rec);
+        lut[dst]     = lut[wpos];
+        const T* src = pts + 3 * static_cast<uint64_t>(wpos);
+        T* d         = pts + 3 * static_cast<uint64_t>(dst);
+        d[0]         = src[0];
+        d[1]         = src[1];
+        d[2]         = src[2];
+        stageNode[dst - N] = ng.id;
+    }
+}
+
+// DFS post-order placement (A6, kernel_helpers.cuh:55-59): subtree sums of subsample sizes
+// bottom-up, then region starts top-down; an internal node's data follows its children's
+// subtrees: dataIdx = N + regionStart + sum of its children's subtree sizes.
+__global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t nodes,
+                                                   const uint8_t* __restrict__ flags,
+                                                   const int32_t* __restrict__ children,
+                                                   const uint32_t* __restrict__ pointCount,
+                                                   uint32_t* __restrict__ subtree)
+{
+    for (uint64_t j = globalThread(); j < nodes; j += globalStride())
+    {
+        uint32_t id = levelStart + static_cast<uint32_t>(j);
+        if (!(flags[id] & NODE_INTERNAL))
+            continue;
+        uint32_t s = pointCount[id];
+        for (int k = 0; k < 8; ++k)
+        {
+            int32_t c = children[8 * static_cast<uint64_t>(id) + k];
+            if (c >= 0)
+                s += subtree[c];
+        }
+        subtree[id] = s;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t nodes, uint32_t root, uint32_t N,
+                                                  const uint8_t* __restrict__ flags,
+                                                  const int32_t* __restrict__ children,
+                                                  const uint32_t* __restrict__ subtree,
+                                                  uint32_t* __restrict__ region, uint64_t* __restrict__ dataIdx)
+{
+    for (uint64_t j = globalThread(); j < nodes; j += globalStride())
+    {
+        uint32_t id = levelStart + static_cast<uint32_t>(j);
+        if (!(flags[id] & NODE_INTERNAL))
+            continue;
+        uint32_t r = (id == root) ? 0u : region[id];
+        for (int k = 0; k < 8; ++k)
+        {
+            int32_t c = children[8 * static_cast<uint64_t>(id) + k];
+            if (c >= 0)
+            {
+                region[c] = r;
+                r += subtree[c];
+            }
+        }
+        dataIdx[id] = static_cast<uint64_t>(N) + r;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kPermute(uint32_t staged, uint32_t N, const uint32_t* __restrict__ stageNode,
+                                                   const uint32_t* __restrict__ stage,
+                                                   const uint64_t* __restrict__ dataIdx,
+                                                   const uint32_t* __restrict__ tmpLut,
+                                                   const OutRecord* __restrict__ tmpOut, uint32_t* __restrict__ lut,
+                                                   OutRecord* __restrict__ out)
+{
+    for (uint64_t s = globalThread(); s < staged; s += globalStride())
+    {
+        uint32_t id  = stageNode[s];
+        uint64_t dst = dataIdx[id] + (static_cast<uint64_t>(N) + s - stage[id]);
+        lut[dst]     = tmpLut[s];
+        storeRecord(out + dst, loadRecord(tmpOut + s));
+    }
+}
+
+}  // namespace oc
