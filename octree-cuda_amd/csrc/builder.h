@@ -161,14 +161,16 @@ private:
     DeviceBuffer itsNodeKey, itsNodeCount, itsNodePoints, itsNodeData, itsNodeParent, itsNodeChild, itsNodeFlags,
             itsNodeFill, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
 
-    // output region [0, E): leaf points, then staged internal subsamples
-    DeviceBuffer itsLut, itsOut, itsPts, itsStageNode, itsTmpLut, itsTmpOut;
+    // output region [0, E) (lut: point index, out: 18-byte record): leaf points at [0, N), internal
+    // subsamples at [N, N + I) after the final placement; pts: leaf coordinates. Staging (st*):
+    // subsampled points in level order, read by the next level, placed at the end.
+    DeviceBuffer itsLut, itsOut, itsPts, itsStageNode, itsStLut, itsStOut, itsStPts;
 
     // per-level subsampling workspace: internal node list, input segments and tiles, per-node
     // occupancy bitmaps with their popcount prefix, per-cell (dense index) state, per-input state
     DeviceBuffer itsInternalList, itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid, itsTileList;
     DeviceBuffer itsBitmap, itsPrefix, itsNodeFirst;
-    DeviceBuffer itsWin, itsColor, itsCellCnt, itsCellStart, itsCellKey;
+    DeviceBuffer itsWin, itsColor, itsCellCnt, itsCellStart, itsCellKey, itsCellNode;
     DeviceBuffer itsCellOf, itsCompactOf, itsRankOf, itsPayload;
 
     // small device scalars and scan scratch

@@ -152,8 +152,13 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
     itsNodeCount.ensure(sizeof(uint32_t) * mBound);
     itsLut.ensure(sizeof(uint32_t) * itsE);
     itsOut.ensure(sizeof(OutRecord) * itsE);
-    itsPts.ensure((cloud.cloudType == CLOUD_FLOAT_UINT8_T ? 12 : 24) * itsE);
-    itsStageNode.ensure(sizeof(uint32_t) * std::max<uint64_t>(1, itsE - cloud.pointAmount));
+    const uint64_t coordBytes = cloud.cloudType == CLOUD_FLOAT_UINT8_T ? 12 : 24;
+    const uint64_t stagedCap  = std::max<uint64_t>(1, itsE - cloud.pointAmount);
+    itsPts.ensure(coordBytes * cloud.pointAmount);
+    itsStageNode.ensure(sizeof(uint32_t) * stagedCap);
+    itsStLut.ensure(sizeof(uint32_t) * stagedCap);
+    itsStOut.ensure(sizeof(OutRecord) * stagedCap);
+    itsStPts.ensure(coordBytes * stagedCap);
     itsScalars.ensure(sizeof(uint64_t) * SC_COUNT);
     OC_HIP_CHECK(hipMemsetAsync(itsOut.get(), 0, sizeof(OutRecord) * itsE, itsStream));
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
@@ -377,23 +382,25 @@ void OctreeBuilder::distributePoints()
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_DIST_END], itsStream));
 }
 
-template <uint32_t MODE>
+template <typename T, uint32_t MODE>
 static void launchAccumulate(bool random, unsigned blocks, hipStream_t s, uint32_t tileBase, uint32_t nodeBase,
                              uint64_t words, const uint32_t* tileList, const uint32_t* segTile,
                              const uint32_t* segCount, const uint32_t* segSrc, const uint32_t* segInput,
                              const NodeGrid* nodeGrid, const uint32_t* bitmap, const uint32_t* prefix,
-                             const uint32_t* cellOf, const uint32_t* lut, const OutRecord* out, uint64_t* win,
-                             uint64_t* color, uint32_t* cellCnt, uint32_t* cellKey, uint32_t* compactOf,
+                             const uint32_t* cellOf, const PointStore<T>& ps, uint64_t* win, uint64_t* color,
+                             uint32_t* cellCnt, uint32_t* cellKey, uint32_t* cellNode, uint32_t* compactOf,
                              uint32_t* rankOf)
 {
     if (random)
-        kAccumulate<MODE, true><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
-                                                         segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, lut, out,
-                                                         win, color, cellCnt, cellKey, compactOf, rankOf);
+        kAccumulate<T, MODE, true><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
+                                                            segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, ps,
+                                                            win, color, cellCnt, cellKey, cellNode, compactOf,
+                                                            rankOf);
     else
-        kAccumulate<MODE, false><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
-                                                          segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, lut,
-                                                          out, win, color, cellCnt, cellKey, compactOf, rankOf);
+        kAccumulate<T, MODE, false><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
+                                                             segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, ps,
+                                                             win, color, cellCnt, cellKey, cellNode, compactOf,
+                                                             rankOf);
 }
 
 // Bytes of bitmap + prefix the nodes of one subsampling batch may use.
@@ -414,6 +421,8 @@ void OctreeBuilder::subsampleLevels()
     const bool packed   = itsColorMode == COLOR_INTRA || itsColorMode == COLOR_INTER;
     const bool weighted = itsColorMode == COLOR_INTER_WEIGHTED;
     uint64_t cursor     = N;
+    PointStore<T> ps{itsLut.as<uint32_t>(),   itsOut.as<OutRecord>(),   itsPts.as<T>(), itsStLut.as<uint32_t>(),
+                     itsStOut.as<OutRecord>(), itsStPts.as<T>(),         N};
 
     SubsampleParams sp{};
     for (int a = 0; a < 3; ++a)
@@ -506,7 +515,7 @@ void OctreeBuilder::subsampleLevels()
                 kMark<T><<<blocks, TILE, 0, itsStream>>>(
                         tileLo, k0, wordsPerNode, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                         itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
-                        itsNodeGrid.as<NodeGrid>(), itsPts.as<T>(), bitmap, itsCellOf.as<uint32_t>(), err);
+                        itsNodeGrid.as<NodeGrid>(), ps, bitmap, itsCellOf.as<uint32_t>(), err);
                 OC_LAUNCH_CHECK();
             }
             scanExclusive<uint32_t>(words, PopcountIn{bitmap}, ExclusiveToArray{prefix},
@@ -525,6 +534,7 @@ void OctreeBuilder::subsampleLevels()
 
             itsWin.ensure(sizeof(uint64_t) * cells);
             itsCellKey.ensure(sizeof(uint32_t) * cells);
+            itsCellNode.ensure(sizeof(uint32_t) * cells);
             if (packed)
                 itsColor.ensure(sizeof(uint64_t) * cells);
             if (weighted)
@@ -543,11 +553,11 @@ void OctreeBuilder::subsampleLevels()
                 switch (itsColorMode)
                 {
 #define OC_ACC(M)                                                                                                      \
-    launchAccumulate<M>(itsProc.useRandomSubsampling, blocks, itsStream, tileLo, k0, wordsPerNode,                    \
-                        itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(), itsSegCount.as<uint32_t>(),            \
-                        itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(), itsNodeGrid.as<NodeGrid>(), bitmap,     \
-                        prefix, itsCellOf.as<uint32_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),              \
-                        itsWin.as<uint64_t>(), color, cellCnt, itsCellKey.as<uint32_t>(),                             \
+    launchAccumulate<T, M>(itsProc.useRandomSubsampling, blocks, itsStream, tileLo, k0, wordsPerNode,                 \
+                           itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(), itsSegCount.as<uint32_t>(),         \
+                           itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(), itsNodeGrid.as<NodeGrid>(), bitmap,  \
+                           prefix, itsCellOf.as<uint32_t>(), ps, itsWin.as<uint64_t>(), color, cellCnt,               \
+                           itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(),                                     \
                         weighted ? itsCompactOf.as<uint32_t>() : nullptr, weighted ? itsRankOf.as<uint32_t>() : nullptr)
                 case COLOR_NONE: OC_ACC(COLOR_NONE); break;
                 case COLOR_INTRA: OC_ACC(COLOR_INTRA); break;
@@ -567,11 +577,18 @@ void OctreeBuilder::subsampleLevels()
                 OC_LAUNCH_CHECK();
                 if (blocks > 0)
                 {
-                    kScatterPayload<T><<<blocks, TILE, 0, itsStream>>>(
-                            tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(), itsSegCount.as<uint32_t>(),
-                            itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(), itsCompactOf.as<uint32_t>(),
-                            itsRankOf.as<uint32_t>(), itsCellStart.as<uint32_t>(), itsPts.as<T>(),
-                            itsOut.as<OutRecord>(), itsPayload.as<Payload<T>>());
+                    if (itsProc.useRandomSubsampling)
+                        kScatterPayload<T, true><<<blocks, TILE, 0, itsStream>>>(
+                                tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
+                                itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
+                                itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
+                                itsCellStart.as<uint32_t>(), ps, itsPayload.as<Payload<T>>());
+                    else
+                        kScatterPayload<T, false><<<blocks, TILE, 0, itsStream>>>(
+                                tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
+                                itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
+                                itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
+                                itsCellStart.as<uint32_t>(), ps, itsPayload.as<Payload<T>>());
                     OC_LAUNCH_CHECK();
                 }
             }
@@ -581,11 +598,10 @@ void OctreeBuilder::subsampleLevels()
             {
 #define OC_EMIT(M)                                                                                                     \
     kEmit<T, M><<<gridFor(cells), BLOCK, 0, itsStream>>>(                                                              \
-            cells, k0, Kb, wordsPerNode, static_cast<uint32_t>(cursor), N, itsNodeFirst.as<uint32_t>(),              \
-            itsNodeGrid.as<NodeGrid>(), bitmap, prefix, itsCellKey.as<uint32_t>(), itsWin.as<uint64_t>(), color,      \
-            cellCnt, weighted ? itsCellStart.as<uint32_t>() : nullptr,                                                \
-            weighted ? itsPayload.as<Payload<T>>() : nullptr, itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),           \
-            itsPts.as<T>(), itsStageNode.as<uint32_t>())
+            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap, prefix,      \
+            itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsWin.as<uint64_t>(), color,                      \
+            weighted ? itsCellStart.as<uint32_t>() : nullptr, weighted ? itsPayload.as<Payload<T>>() : nullptr, ps,   \
+            itsStageNode.as<uint32_t>())
             case COLOR_NONE: OC_EMIT(COLOR_NONE); break;
             case COLOR_INTRA: OC_EMIT(COLOR_INTRA); break;
             case COLOR_INTER: OC_EMIT(COLOR_INTER); break;
@@ -628,16 +644,10 @@ void OctreeBuilder::placeInternalNodes()
     }
     if (staged > 0)
     {
-        itsTmpLut.ensure(sizeof(uint32_t) * staged);
-        itsTmpOut.ensure(sizeof(OutRecord) * staged);
-        OC_HIP_CHECK(hipMemcpyAsync(itsTmpLut.get(), itsLut.as<uint32_t>() + N, sizeof(uint32_t) * staged,
-                                    hipMemcpyDeviceToDevice, itsStream));
-        OC_HIP_CHECK(hipMemcpyAsync(itsTmpOut.get(), itsOut.as<OutRecord>() + N, sizeof(OutRecord) * staged,
-                                    hipMemcpyDeviceToDevice, itsStream));
         kPermute<<<gridFor(staged), BLOCK, 0, itsStream>>>(
                 static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
-                itsNodeData.as<uint64_t>(), itsTmpLut.as<uint32_t>(), itsTmpOut.as<OutRecord>(),
-                itsLut.as<uint32_t>(), itsOut.as<OutRecord>());
+                itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(), itsLut.as<uint32_t>(),
+                itsOut.as<OutRecord>());
         OC_LAUNCH_CHECK();
     }
 }

@@ -25,6 +25,30 @@ namespace oc {
 
 constexpr uint32_t TILE = 256;  // inputs per workgroup in the per-input kernels
 
+// The points a subsampling level reads: leaf points at positions [0, N) (in their final place in
+// the output) and already subsampled points of lower levels at positions [N, N + I) (staging, in
+// level order until the final DFS post-order placement).
+template <typename T>
+struct PointStore
+{
+    uint32_t* lut;
+    OutRecord* out;
+    T* pts;
+    uint32_t* stLut;
+    OutRecord* stOut;
+    T* stPts;
+    uint32_t N;
+    __device__ __forceinline__ uint32_t index(uint32_t pos) const { return pos < N ? lut[pos] : stLut[pos - N]; }
+    __device__ __forceinline__ OutRecord record(uint32_t pos) const
+    {
+        return loadRecord(pos < N ? out + pos : stOut + (pos - N));
+    }
+    __device__ __forceinline__ const T* xyz(uint32_t pos) const
+    {
+        return pos < N ? pts + 3 * static_cast<uint64_t>(pos) : stPts + 3 * static_cast<uint64_t>(pos - N);
+    }
+};
+
 struct SubsampleParams
 {
     double cubicMin[3];
@@ -181,7 +205,7 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
                                                const uint32_t* __restrict__ segCount,
                                                const uint32_t* __restrict__ segSrc,
                                                const uint32_t* __restrict__ segInput,
-                                               const NodeGrid* __restrict__ nodeGrid, const T* __restrict__ pts,
+                                               const NodeGrid* __restrict__ nodeGrid, PointStore<T> ps,
                                                uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
                                                uint32_t* __restrict__ err)
 {
@@ -189,7 +213,7 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
     if (!in.valid)
         return;
     const NodeGrid ng = nodeGrid[in.k];
-    const T* p        = pts + 3 * static_cast<uint64_t>(in.pos);
+    const T* p        = ps.xyz(in.pos);
     uint64_t ix, iy, iz;
     gridCoords(static_cast<double>(p[0]), static_cast<double>(p[1]), static_cast<double>(p[2]), ng.grid, ix, iy, iz);
     const uint32_t gs = ng.grid.gridSize;
@@ -254,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __r
 // for the implicit method or the keyed bijective hash of it for the explicit one), the packed
 // colour sum of the input records (intra-cell, and the per-cell sums the inter-cell filter is
 // built from), and per-cell point counts + each point's rank in its cell (weighted filter).
-template <uint32_t MODE, bool RANDOM>
+template <typename T, uint32_t MODE, bool RANDOM>
 __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
                                                      const uint32_t* __restrict__ tileList,
                                                      const uint32_t* __restrict__ segTile,
@@ -264,12 +288,11 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
                                                      const NodeGrid* __restrict__ nodeGrid,
                                                      const uint32_t* __restrict__ bitmap,
                                                      const uint32_t* __restrict__ prefix,
-                                                     const uint32_t* __restrict__ cellOf,
-                                                     const uint32_t* __restrict__ lut,
-                                                     const OutRecord* __restrict__ out, uint64_t* __restrict__ win,
+                                                     const uint32_t* __restrict__ cellOf, PointStore<T> ps,
+                                                     uint64_t* __restrict__ win,
                                                      uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
-                                                     uint32_t* __restrict__ cellKey, uint32_t* __restrict__ compactOf,
-                                                     uint32_t* __restrict__ rankOf)
+                                                     uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
+                                                     uint32_t* __restrict__ compactOf, uint32_t* __restrict__ rankOf)
 {
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
     if (!in.valid)
@@ -279,109 +302,100 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
     if (!((bitmap[word] >> (cell & 31)) & 1u))
         return;  // the point lay outside its node's grid (reported by kMark)
     const uint32_t c = cellRank(bitmap, prefix, word, cell & 31);
-    const uint32_t idx  = lut[in.pos];
+    // every point of the cell stores the same key and node
+    __hip_atomic_store(cellKey + c, cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(cellNode + c, in.k - nodeBase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (MODE == COLOR_INTER_WEIGHTED)
+    {
+        // the winner is taken from the cell's payload run in the emit
+        compactOf[in.q] = c;
+        rankOf[in.q]    = atomicAdd(cellCnt + c, 1u);
+        return;
+    }
+    const uint32_t idx  = ps.index(in.pos);
     const uint32_t rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
     atomicMin(reinterpret_cast<unsigned long long*>(win + c),
               static_cast<unsigned long long>((static_cast<uint64_t>(rank) << 32) | in.pos));
-    // every point of the cell stores the same key
-    __hip_atomic_store(cellKey + c, cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
     {
-        OutRecord r = loadRecord(out + in.pos);
+        OutRecord r = ps.record(in.pos);
         atomicAdd(reinterpret_cast<unsigned long long*>(color + c),
                   static_cast<unsigned long long>(encodeColors(r.r, r.g, r.b)));
-    }
-    if (MODE == COLOR_INTER_WEIGHTED)
-    {
-        compactOf[in.q] = c;
-        rankOf[in.q]    = atomicAdd(cellCnt + c, 1u);
     }
 }
 
 // Point payload of the weighted filter, stored in cell order so that a cell's points are one
-// contiguous run (coordinates in the cloud's type and the input record colour).
+// contiguous run: coordinates in the cloud's type, the winner rank and input position, and the
+// input record colour.
 template <typename T>
-struct alignas(sizeof(T) == 8 ? 16 : 4) Payload
+struct alignas(8) Payload
 {
     T x, y, z;
+    uint32_t rank, pos;
     uint16_t r, g, b, pad;
 };
 
-template <typename T>
+template <typename T, bool RANDOM>
 __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
                                                          const uint32_t* __restrict__ segTile,
                                                          const uint32_t* __restrict__ segCount,
                                                          const uint32_t* __restrict__ segSrc,
                                                          const uint32_t* __restrict__ segInput,
+                                                         const NodeGrid* __restrict__ nodeGrid,
                                                          const uint32_t* __restrict__ compactOf,
                                                          const uint32_t* __restrict__ rankOf,
-                                                         const uint32_t* __restrict__ cellStart,
-                                                         const T* __restrict__ pts, const OutRecord* __restrict__ out,
+                                                         const uint32_t* __restrict__ cellStart, PointStore<T> ps,
                                                          Payload<T>* __restrict__ payload)
 {
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
     if (!in.valid)
         return;
-    const T* p  = pts + 3 * static_cast<uint64_t>(in.pos);
-    OutRecord r = loadRecord(out + in.pos);
+    const T* p         = ps.xyz(in.pos);
+    OutRecord r        = ps.record(in.pos);
+    const uint32_t idx = ps.index(in.pos);
     Payload<T> pl;
-    pl.x   = p[0];
-    pl.y   = p[1];
-    pl.z   = p[2];
-    pl.r   = r.r;
-    pl.g   = r.g;
-    pl.b   = r.b;
-    pl.pad = 0;
+    pl.x    = p[0];
+    pl.y    = p[1];
+    pl.z    = p[2];
+    pl.rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
+    pl.pos  = in.pos;
+    pl.r    = r.r;
+    pl.g    = r.g;
+    pl.b    = r.b;
+    pl.pad  = 0;
     payload[cellStart[compactOf[in.q]] + rankOf[in.q]] = pl;
 }
 
-// node of dense index c: last kl with nodeFirst[kl] <= c (nodes without cells share the next
-// node's first index)
-__device__ __forceinline__ uint32_t nodeOfCell(const uint32_t* nodeFirst, uint32_t nodes, uint32_t c)
-{
-    uint32_t lo = 0, hi = nodes;
-    while (hi - lo > 1)
-    {
-        uint32_t mid = (lo + hi) >> 1;
-        if (nodeFirst[mid] <= c)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Select (rp_subsampling.cuh:70-321, fp_subsampling.cuh:30-306): one output record per occupied
-// cell, written at cursor + its dense index (node by node, cells ascending). The record keeps the
+// cell, written to staging at its dense index (node by node, cells ascending). The record keeps the
 // winner's global quantised position; the colour is the winner's (no filter), the decoded intra
 // sum, the decoded sum of the per-cell sums over the 3x3x3 neighbourhood (inter; equal to the
 // reference's per-point scatter into occupied neighbours because integer sums commute), or the
-// gathered distance-weighted average (kernel_inter_cell_filtering.cuh:109-194) over the points
-// of the neighbourhood, with the weight of a point p for this cell c:
+// gathered distance-weighted average (kernel_inter_cell_filtering.cuh:109-194) over the points of
+// the neighbourhood, with the weight of a point p for this cell c:
 //   res = float(sqrt(|centre(c) - p|^2) / diag);  w = expf(-(res * res) * 10).
+// The neighbourhood is walked as 9 x-rows: the occupied cells among x-1..x+1 of one row have
+// consecutive dense indices, so each row is one contiguous run of per-cell sums or payloads.
 template <typename T, uint32_t MODE>
-__global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint32_t nodes,
-                                                uint64_t wordsPerNode, uint32_t cursor, uint32_t N,
-                                                const uint32_t* __restrict__ nodeFirst,
-                                                const NodeGrid* __restrict__ nodeGrid,
+__global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
                                                 const uint32_t* __restrict__ bitmap,
                                                 const uint32_t* __restrict__ prefix,
                                                 const uint32_t* __restrict__ cellKey,
+                                                const uint32_t* __restrict__ cellNode,
                                                 const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
-                                                const uint32_t* __restrict__ cellCnt,
                                                 const uint32_t* __restrict__ cellStart,
-                                                const Payload<T>* __restrict__ payload, uint32_t* __restrict__ lut,
-                                                OutRecord* __restrict__ out, T* __restrict__ pts,
+                                                const Payload<T>* __restrict__ payload, PointStore<T> ps,
                                                 uint32_t* __restrict__ stageNode)
 {
     for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
     {
-        const uint32_t c    = static_cast<uint32_t>(ci);
-        const uint32_t kl   = nodeOfCell(nodeFirst, nodes, c);
-        const NodeGrid ng   = nodeGrid[nodeBase + kl];
-        const uint32_t wpos = static_cast<uint32_t>(win[c] & 0xFFFFFFFFull);
-        OutRecord rec       = loadRecord(out + wpos);
-        const uint32_t dst  = cursor + c;
+        const uint32_t c  = static_cast<uint32_t>(ci);
+        const uint32_t kl = cellNode[c];
+        const NodeGrid ng = nodeGrid[nodeBase + kl];
+        uint64_t best     = MODE == COLOR_INTER_WEIGHTED ? ~0ull : win[c];
+        OutRecord rec{};
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
 
         if (MODE == COLOR_INTRA)
         {
@@ -389,74 +403,110 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
         }
         else if (MODE == COLOR_INTER || MODE == COLOR_INTER_WEIGHTED)
         {
-            const uint32_t key   = cellKey[c];
-            const uint32_t gs    = ng.grid.gridSize;
-            const uint64_t words = static_cast<uint64_t>(kl) * wordsPerNode;
+            const uint32_t key    = cellKey[c];
+            const uint32_t gs     = ng.grid.gridSize;
+            const uint64_t words  = static_cast<uint64_t>(kl) * wordsPerNode;
             const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>((key / gs) % gs),
                           cz = static_cast<int32_t>(key / (gs * gs));
-            uint64_t sum   = 0;
-            float acc[4]   = {0.f, 0.f, 0.f, 0.f};
-            const double t = ng.grid.bbSize / gs;
-            const double ctrX = ng.grid.minX + cx * t + (t * 0.5);
-            const double ctrY = ng.grid.minY + cy * t + (t * 0.5);
-            const double ctrZ = ng.grid.minZ + cz * t + (t * 0.5);
-            for (int ox = -1; ox <= 1; ++ox)
-                for (int oy = -1; oy <= 1; ++oy)
-                    for (int oz = -1; oz <= 1; ++oz)
-                    {
-                        const int32_t nx = cx + ox, ny = cy + oy, nz = cz + oz;
-                        if (nx < 0 || ny < 0 || nz < 0 || nx >= static_cast<int32_t>(gs) ||
-                            ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
-                            continue;
-                        const uint32_t ncell = static_cast<uint32_t>(nx) + static_cast<uint32_t>(ny) * gs +
-                                               static_cast<uint32_t>(nz) * gs * gs;
-                        const uint64_t w  = words + (ncell >> 5);
-                        const uint32_t bw = bitmap[w];
-                        if (!((bw >> (ncell & 31)) & 1u))
-                            continue;
-                        const uint32_t nc = prefix[w] + __popc(bw & ((1u << (ncell & 31)) - 1u));
-                        if (MODE == COLOR_INTER)
-                        {
-                            sum += color[nc];
-                        }
-                        else
-                        {
-                            const uint32_t first = cellStart[nc];
-                            const uint32_t cnt   = cellCnt[nc];
-                            for (uint32_t q = 0; q < cnt; ++q)
-                            {
-                                const Payload<T> pl = payload[first + q];
-                                const double uX     = ctrX - static_cast<double>(pl.x);
-                                const double uY     = ctrY - static_cast<double>(pl.y);
-                                const double uZ     = ctrZ - static_cast<double>(pl.z);
-                                float res = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
-                                res       = expf(-((res * res) * 10.f));
-                                acc[0] += static_cast<float>(pl.r) * res;
-                                acc[1] += static_cast<float>(pl.g) * res;
-                                acc[2] += static_cast<float>(pl.b) * res;
-                                acc[3] += res;
-                            }
-                        }
-                    }
+            const int32_t x0 = cx > 0 ? cx - 1 : 0;
+            const int32_t x1 = cx + 1 < static_cast<int32_t>(gs) ? cx + 1 : cx;
+            const uint32_t span = static_cast<uint32_t>(x1 - x0 + 1);
+
+            // pass 1: first dense index and number of occupied cells of every row
+            uint32_t first[9], count[9];
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+            {
+                const int32_t ny = cy + (r / 3) - 1, nz = cz + (r % 3) - 1;
+                count[r] = 0;
+                first[r] = 0;
+                if (ny < 0 || nz < 0 || ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
+                    continue;
+                const uint64_t L0 = static_cast<uint64_t>(x0) + static_cast<uint64_t>(ny) * gs +
+                                    static_cast<uint64_t>(nz) * gs * gs;
+                const uint64_t w0 = words + (L0 >> 5);
+                const uint32_t b0 = static_cast<uint32_t>(L0 & 31);
+                uint64_t v        = bitmap[w0];
+                if (b0 + span > 32)
+                    v |= static_cast<uint64_t>(bitmap[w0 + 1]) << 32;
+                const uint32_t m = static_cast<uint32_t>(v >> b0) & ((1u << span) - 1u);
+                count[r]         = __popc(m);
+                first[r]         = prefix[w0] + __popc(static_cast<uint32_t>(v) & ((1u << b0) - 1u));
+            }
             if (MODE == COLOR_INTER)
             {
+                uint64_t sum = 0;
+#pragma unroll
+                for (int r = 0; r < 9; ++r)
+                    for (uint32_t i = 0; i < count[r]; ++i)
+                        sum += color[first[r] + i];
                 decodeColors(sum, rec.r, rec.g, rec.b);
             }
             else
+            {
+                // pass 2: payload runs of the rows
+                uint32_t lo[9], hi[9];
+#pragma unroll
+                for (int r = 0; r < 9; ++r)
+                {
+                    lo[r] = count[r] ? cellStart[first[r]] : 0;
+                    hi[r] = count[r] ? cellStart[first[r] + count[r]] : 0;
+                }
+                const uint32_t cLo = cellStart[c], cHi = cellStart[c + 1];
+                const double t     = ng.grid.bbSize / gs;
+                const double ctrX  = ng.grid.minX + cx * t + (t * 0.5);
+                const double ctrY  = ng.grid.minY + cy * t + (t * 0.5);
+                const double ctrZ  = ng.grid.minZ + cz * t + (t * 0.5);
+                for (int r = 0; r < 9; ++r)
+                {
+                    for (uint32_t q = lo[r]; q < hi[r]; ++q)
+                    {
+                        const Payload<T> pl = payload[q];
+                        if (q >= cLo && q < cHi)
+                        {
+                            const uint64_t k2 = (static_cast<uint64_t>(pl.rank) << 32) | pl.pos;
+                            best              = k2 < best ? k2 : best;
+                        }
+                        const double uX = ctrX - static_cast<double>(pl.x);
+                        const double uY = ctrY - static_cast<double>(pl.y);
+                        const double uZ = ctrZ - static_cast<double>(pl.z);
+                        float res       = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
+                        res             = expf(-((res * res) * 10.f));
+                        acc[0] += static_cast<float>(pl.r) * res;
+                        acc[1] += static_cast<float>(pl.g) * res;
+                        acc[2] += static_cast<float>(pl.b) * res;
+                        acc[3] += res;
+                    }
+                }
+            }
+        }
+        const uint32_t wpos = static_cast<uint32_t>(best & 0xFFFFFFFFull);
+        OutRecord wrec      = ps.record(wpos);
+        if (MODE == COLOR_NONE)
+        {
+            rec = wrec;
+        }
+        else
+        {
+            rec.x = wrec.x;
+            rec.y = wrec.y;
+            rec.z = wrec.z;
+            if (MODE == COLOR_INTER_WEIGHTED)
             {
                 rec.r = static_cast<uint16_t>(acc[0] / acc[3]);
                 rec.g = static_cast<uint16_t>(acc[1] / acc[3]);
                 rec.b = static_cast<uint16_t>(acc[2] / acc[3]);
             }
         }
-        storeRecord(out + dst, rec);
-        lut[dst]     = lut[wpos];
-        const T* src = pts + 3 * static_cast<uint64_t>(wpos);
-        T* d         = pts + 3 * static_cast<uint64_t>(dst);
+        const uint32_t st = cursor - ps.N + c;  // staging index
+        storeRecord(ps.stOut + st, rec);
+        ps.stLut[st] = ps.index(wpos);
+        const T* src = ps.xyz(wpos);
+        T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
         d[0]         = src[0];
         d[1]         = src[1];
         d[2]         = src[2];
-        stageNode[dst - N] = ng.id;
+        stageNode[st] = ng.id;
     }
 }
 
@@ -513,16 +563,16 @@ __global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t n
 __global__ __launch_bounds__(BLOCK) void kPermute(uint32_t staged, uint32_t N, const uint32_t* __restrict__ stageNode,
                                                    const uint32_t* __restrict__ stage,
                                                    const uint64_t* __restrict__ dataIdx,
-                                                   const uint32_t* __restrict__ tmpLut,
-                                                   const OutRecord* __restrict__ tmpOut, uint32_t* __restrict__ lut,
+                                                   const uint32_t* __restrict__ stLut,
+                                                   const OutRecord* __restrict__ stOut, uint32_t* __restrict__ lut,
                                                    OutRecord* __restrict__ out)
 {
     for (uint64_t s = globalThread(); s < staged; s += globalStride())
     {
         uint32_t id  = stageNode[s];
         uint64_t dst = dataIdx[id] + (static_cast<uint64_t>(N) + s - stage[id]);
-        lut[dst]     = tmpLut[s];
-        storeRecord(out + dst, loadRecord(tmpOut + s));
+        lut[dst]     = stLut[s];
+        storeRecord(out + dst, loadRecord(stOut + s));
     }
 }
 
