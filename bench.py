@@ -34,13 +34,36 @@ def log(*a):
 
 
 def algorithmic_bytes(kind, n, stride, grid, summ):
-    """Compulsory DRAM bytes per launch of each kernel class (one input point = one unit)."""
+    """Compulsory HBM bytes of one build per stage (one input point = one unit):
+    count       N*S + 4*G^3                   read every record, write the dense histogram
+    distribute  N*(S + 4 + 4 + 18)            record, cell -> leaf lookup, index write, 18-byte record write
+    subsample   C*(4 + S + 6) + I*(4 + 18)    C = N + I - I_root candidate points (index, record, colour),
+                                              I subsampled points (index + record written)"""
     internal_points = summ["used"] - n
-    if kind == "count":  # read every record, write the dense histogram
+    if kind == "count":
         return n * stride + 4 * grid ** 3
-    if kind == "distribute":  # record, cell -> leaf lookup, index write, 18-byte record write
+    if kind == "distribute":
         return n * (stride + 4 + 4 + 18)
-    return None
+    if kind == "subsample":
+        candidates = n + internal_points - summ["root_points"]
+        return candidates * (4 + stride + 6) + internal_points * (4 + 18)
+    raise ValueError(kind)
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size (for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"bench: copied {nbytes} bytes device to device 3 times (counter calibration)")
 
 
 def cpu_baseline(cloud, wl, cubic, sample_points, threads):
@@ -56,8 +79,8 @@ def cpu_baseline(cloud, wl, cubic, sample_points, threads):
     if r.status != 0:
         return None
     return {"value": sample_points / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (std::thread x {threads}) on the first {sample_points:,} points of the same "
-                      f"cloud with the same bounding box and configuration, {dt:.1f} s"}
+            "sample": f"CPU oracle (std::thread x {threads}) building the first {sample_points:,} points of the "
+                      f"same cloud with the same bounding box and configuration once, {dt:.1f} s"}
 
 
 def main():
@@ -66,8 +89,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="C3", choices=sorted(W.WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=10_000_000, help="points for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="points of the cloud the CPU oracle baseline builds (-1 = all, 0 = skip)")
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, help="also run a plain D2D copy of this many bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -103,28 +128,39 @@ def main():
         timings.append(sess.timings())
         log(f"bench: step {i} {step_ms[-1]:.2f} ms {timings[-1]}")
     summ = sess.summary()
+    nodes = sess.nodes()
+    summ["root_points"] = int(nodes.point_count[-1]) if nodes.internal[-1] else 0
     ms = float(np.mean(step_ms))
     avg = {k: float(np.mean([t[k] for t in timings])) for k in timings[0]}
 
-    # dominant kernel class and its roofline
-    classes = {"count": avg["count_ms"], "distribute": avg["distribute_ms"],
-               "subsample_insert": avg["insert_ms"], "subsample_emit": avg["emit_ms"]}
-    dominant = max(classes, key=classes.get)
-    rl_kind = dominant if dominant in ("count", "distribute") else max(("count", "distribute"), key=classes.get)
-    rl_bytes = algorithmic_bytes(rl_kind, s.points, s.stride, wl.build.chunking_grid, summ)
-    rl_ms = classes[rl_kind]
-    achieved = rl_bytes / (rl_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": rl_kind, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    # roofline of the dominant stage (device time from HIP events on the session stream)
+    stage_ms = {"count": avg["count_ms"], "distribute": avg["distribute_ms"], "subsample": avg["subsample_ms"]}
+    grid = wl.build.chunking_grid
+
+    def rl(kind):
+        b = algorithmic_bytes(kind, s.points, s.stride, grid, summ)
+        gbs = b / (stage_ms[kind] * 1e-3) / 1e9
+        return b, gbs
+
+    dominant = max(stage_ms, key=stage_ms.get)
+    rl_bytes, achieved = rl(dominant)
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": rl_bytes, "avg_launch_ms": round(rl_ms, 4),
-                "dominant_kernel_class": dominant,
-                "kernel_class_ms": {k: round(v, 4) for k, v in classes.items()}}
+                "bytes_per_launch": rl_bytes, "avg_launch_ms": round(stage_ms[dominant], 4),
+                "stages": {k: {"ms": round(v, 4), "bytes": rl(k)[0], "GBs": round(rl(k)[1], 1),
+                               "frac": round(rl(k)[1] / HBM_PEAK_GBS, 4)} for k, v in stage_ms.items()},
+                "whole_build": {"bytes": sum(rl(k)[0] for k in stage_ms) + 4 * summ["dense_nodes"],
+                                "GBs": round((sum(rl(k)[0] for k in stage_ms) + 4 * summ["dense_nodes"])
+                                             / (avg["build_ms"] * 1e-3) / 1e9, 1)}}
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample != 0:
         threads = min(os.cpu_count() or 1, 16)
-        log(f"bench: CPU oracle baseline on {args.cpu_sample:,} points with {threads} threads")
-        cpu = cpu_baseline(cloud, wl, cubic, min(args.cpu_sample, s.points), threads)
+        sample = s.points if args.cpu_sample < 0 else min(args.cpu_sample, s.points)
+        log(f"bench: CPU oracle baseline on {sample:,} points with {threads} threads")
+        cpu = cpu_baseline(cloud, wl, cubic, sample, threads)
 
     line = {
         "metric": "points/sec full octree build (1 MI355X, % HBM roofline)",
