@@ -2,14 +2,17 @@
 //
 // Replaces the reference's OctreeProcessorImpl (octree_processor_impl.cu:12-186,
 // octree_chunking.cu:18-124, rp_subsampling.cu, fp_subsampling.cu). Phases:
-//   count       one pass over the cloud: dense level-0 histogram (K1)
-//   merge       count pyramid, canonical sparse ids by scan, node links and flags, leaf data
-//               blocks by scan (K2-K4), cell -> leaf map
-//   distribute  one pass over the cloud: slot per point in its leaf, point index, 18-byte
-//               record and the coordinates for the subsampling (K5)
+//   count       two-pass Morton bucket sort of the cloud (coarse LDS histograms, bucket scatter,
+//               per-bucket LDS fine sort) that also yields the dense count pyramid up to the
+//               bucket level (K1)
+//   merge       rest of the count pyramid, canonical sparse ids by scan, node links and flags,
+//               leaf data blocks by scan (K2-K4), cell -> leaf map
+//   distribute  each leaf's range of the sorted cloud -> its data block: point index and
+//               18-byte record, coalesced (K5)
 //   subsample   one batch per octree level (all internal nodes of a level at once) with per-node
-//               open-addressing cell tables; winners by atomicMin, colours by packed integer
-//               sums or a gathered distance-weighted filter; then DFS post-order placement
+//               occupancy bitmaps and a popcount rank dictionary; winners by atomicMin, colours
+//               by packed integer sums or a gathered distance-weighted filter; then DFS
+//               post-order placement
 // All buffers live in a grow-only per-session arena, so repeated builds allocate nothing.
 #pragma once
 
@@ -139,6 +142,8 @@ private:
     // level tables (octree.cu:13-23)
     uint32_t itsDepth = 0;
     uint32_t itsDense = 0;
+    uint32_t itsCoarse  = 0;  // bucket level of the sort
+    uint32_t itsBuckets = 0;  // (chunkingGrid >> itsCoarse)^3
     uint32_t itsLevelGrid[MAX_LEVELS]   = {};
     uint32_t itsLevelOffset[MAX_LEVELS] = {};
     uint32_t itsLevelNodes[MAX_LEVELS]  = {};
@@ -159,12 +164,15 @@ private:
     // dense pyramid and sparse nodes
     DeviceBuffer itsCount, itsDenseToSparse;
     DeviceBuffer itsNodeKey, itsNodeCount, itsNodePoints, itsNodeData, itsNodeParent, itsNodeChild, itsNodeFlags,
-            itsNodeFill, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
+            itsNodeSorted, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
+
+    // Morton bucket sort of the cloud
+    DeviceBuffer itsBucketed, itsSorted, itsFineCode, itsBucketCount, itsBucketStart, itsBucketCursor;
 
     // output region [0, E) (lut: point index, out: 18-byte record): leaf points at [0, N), internal
-    // subsamples at [N, N + I) after the final placement; pts: leaf coordinates. Staging (st*):
-    // subsampled points in level order, read by the next level, placed at the end.
-    DeviceBuffer itsLut, itsOut, itsPts, itsStageNode, itsStLut, itsStOut, itsStPts;
+    // subsamples at [N, N + I) after the final placement. Staging (st*): subsampled points in
+    // level order, read by the next level, placed at the end.
+    DeviceBuffer itsLut, itsOut, itsStageNode, itsStLut, itsStOut, itsStPts;
 
     // per-level subsampling workspace: internal node list, input segments and tiles, per-node
     // occupancy bitmaps with their popcount prefix, per-cell (dense index) state, per-input state

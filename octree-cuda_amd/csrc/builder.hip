@@ -1,15 +1,19 @@
 // Level-batched LOD octree build on one MI355X (gfx950). See builder.h for the phase overview.
 //
 // Data layout in HBM (per session):
-//   cloud          N x S bytes, packed AoS records (15 or 27 bytes), read twice (count, distribute)
+//   cloud          N x S bytes, packed AoS records (15 or 27 bytes), read twice (bucket
+//                  histogram, bucket scatter)
+//   bucketed       N sorted-point records (32 bytes for double clouds, 20 for float) grouped by
+//                  coarse bucket, plus a u16 fine code per point
+//   sorted         the same records in level-0 Morton order: every node is one contiguous range;
+//                  the leaves' points are read from here by the subsampling
 //   count          D x u32 dense count pyramid; after the merge its level-0 part maps cell -> leaf
 //   denseToSparse  D x i32
 //   nodes (SoA)    key, dense count, pointCount, dataIdx (u64), parent, children[8], flags, ...
-//   output region  E records: lut (u32 point index), out (18-byte record), pts (3 x coordinate
-//                  type, the original coordinates). [0, N) holds the leaves, [N, N + I) the
-//                  subsampled points, first in level order (staging) and finally in DFS
-//                  post-order.
-//   level tables   per subsampling level: open-addressing cell tables, one slice per internal node
+//   output region  E records: lut (u32 point index), out (18-byte record). [0, N) holds the leaves,
+//                  [N, N + I) the subsampled points in DFS post-order
+//   staging        subsampled points in level order (index, record, coordinates), read by the
+//                  next level and permuted into the output region at the end
 #include "builder.h"
 
 #include <hip/hip_runtime.h>
@@ -23,6 +27,7 @@
 
 #include "device_common.h"
 #include "kernels_chunking.h"
+#include "kernels_sort.h"
 #include "kernels_subsample.h"
 #include "scan.h"
 
@@ -121,6 +126,8 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
         itsLevelNodes[l]    = g * g * g;
         itsDense += g * g * g;
     }
+    itsCoarse  = std::min<uint32_t>(MAX_COARSE_LEVEL, itsDepth > 3 ? itsDepth - 3 : 0);
+    itsBuckets = (proc.chunkingGrid >> itsCoarse) * (proc.chunkingGrid >> itsCoarse) * (proc.chunkingGrid >> itsCoarse);
     itsColorMode = proc.useIntraCellAvg ? COLOR_INTRA
                                         : (proc.useInterCellAvg ? (proc.useWeightingFunction ? COLOR_INTER_WEIGHTED
                                                                                             : COLOR_INTER)
@@ -152,9 +159,16 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
     itsNodeCount.ensure(sizeof(uint32_t) * mBound);
     itsLut.ensure(sizeof(uint32_t) * itsE);
     itsOut.ensure(sizeof(OutRecord) * itsE);
-    const uint64_t coordBytes = cloud.cloudType == CLOUD_FLOAT_UINT8_T ? 12 : 24;
+    const bool isFloat        = cloud.cloudType == CLOUD_FLOAT_UINT8_T;
+    const uint64_t coordBytes = isFloat ? 12 : 24;
+    const uint64_t sortedBytes = isFloat ? sizeof(SortedPoint<float>) : sizeof(SortedPoint<double>);
     const uint64_t stagedCap  = std::max<uint64_t>(1, itsE - cloud.pointAmount);
-    itsPts.ensure(coordBytes * cloud.pointAmount);
+    itsSorted.ensure(sortedBytes * cloud.pointAmount);
+    itsBucketed.ensure(sortedBytes * cloud.pointAmount);
+    itsFineCode.ensure(sizeof(uint16_t) * cloud.pointAmount);
+    itsBucketCount.ensure(sizeof(uint32_t) * (itsBuckets + 1));
+    itsBucketStart.ensure(sizeof(uint32_t) * (itsBuckets + 1));
+    itsBucketCursor.ensure(sizeof(uint32_t) * (itsBuckets + 1));
     itsStageNode.ensure(sizeof(uint32_t) * stagedCap);
     itsStLut.ensure(sizeof(uint32_t) * stagedCap);
     itsStOut.ensure(sizeof(OutRecord) * stagedCap);
@@ -265,16 +279,44 @@ void OctreeBuilder::generateTyped()
 template <typename T>
 void OctreeBuilder::countPoints()
 {
-    uint64_t* sc    = itsScalars.as<uint64_t>();
-    uint32_t* count = itsCount.as<uint32_t>();
-    kFill32<<<gridFor(itsLevelNodes[0]), BLOCK, 0, itsStream>>>(count, itsLevelNodes[0], 0);
-    OC_LAUNCH_CHECK();
+    uint64_t* sc       = itsScalars.as<uint64_t>();
+    const uint32_t n   = itsCloud.pointAmount;
+    const uint32_t nb  = itsBuckets;
+    uint32_t* bucketCount  = itsBucketCount.as<uint32_t>();
+    uint32_t* bucketStart  = itsBucketStart.as<uint32_t>();
+    uint32_t* bucketCursor = itsBucketCursor.as<uint32_t>();
     Gridding g{itsCloud.bbMin[0], itsCloud.bbMin[1], itsCloud.bbMin[2], itsCloud.bbMax[0] - itsCloud.bbMin[0],
                itsProc.chunkingGrid};
+    itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(nb) + 1));
+    kFill32<<<gridFor(nb + 1), BLOCK, 0, itsStream>>>(bucketCount, nb + 1, 0);
+    OC_LAUNCH_CHECK();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_BEGIN], itsStream));
-    kCount<T><<<gridFor(itsCloud.pointAmount, 16384), BLOCK, 0, itsStream>>>(
-            itsCloudDevice, itsCloud.pointAmount, itsCloud.dataStride, g, count,
-            reinterpret_cast<uint32_t*>(sc + SC_ERROR));
+
+    const unsigned histBlocks = static_cast<unsigned>(std::min<uint64_t>(2048, (n + 8191) / 8192));
+    kBucketHist<T><<<histBlocks, HIST_THREADS, 0, itsStream>>>(itsCloudDevice, n, itsCloud.dataStride, g,
+                                                               3 * itsCoarse, nb, bucketCount,
+                                                               reinterpret_cast<uint32_t*>(sc + SC_ERROR));
+    OC_LAUNCH_CHECK();
+    scanExclusive<uint32_t>(nb, ArrayIn{bucketCount}, BucketStartOut{bucketStart, bucketCursor},
+                            itsScanScratch.as<uint32_t>(), bucketStart + nb, itsStream);
+    OC_LAUNCH_CHECK();
+    const unsigned scatterBlocks = static_cast<unsigned>((static_cast<uint64_t>(n) + SCATTER_CHUNK - 1) / SCATTER_CHUNK);
+    kBucketScatter<T><<<scatterBlocks, SORT_THREADS, 0, itsStream>>>(itsCloudDevice, n, itsCloud.dataStride, g,
+                                                                     3 * itsCoarse, nb, bucketCursor,
+                                                                     itsBucketed.as<SortedPoint<T>>(),
+                                                                     itsFineCode.as<uint16_t>());
+    OC_LAUNCH_CHECK();
+    FineSortParams fp{};
+    fp.lc         = itsCoarse;
+    fp.bucketGrid = itsProc.chunkingGrid >> itsCoarse;
+    for (uint32_t l = 0; l <= itsDepth; ++l)
+    {
+        fp.levelOffset[l] = itsLevelOffset[l];
+        fp.levelGrid[l]   = itsLevelGrid[l];
+    }
+    kFineSort<T><<<nb, SORT_THREADS, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(), itsFineCode.as<uint16_t>(),
+                                                     bucketStart, fp, itsCount.as<uint32_t>(),
+                                                     itsSorted.as<SortedPoint<T>>());
     OC_LAUNCH_CHECK();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));
 }
@@ -283,7 +325,7 @@ void OctreeBuilder::mergeLevels()
 {
     uint64_t* sc    = itsScalars.as<uint64_t>();
     uint32_t* count = itsCount.as<uint32_t>();
-    for (uint32_t l = 1; l <= itsDepth; ++l)
+    for (uint32_t l = itsCoarse + 1; l <= itsDepth; ++l)  // levels 0..coarse come from the fine sort
     {
         kPropagate<<<gridFor(itsLevelNodes[l]), BLOCK, 0, itsStream>>>(count, itsLevelOffset[l], itsLevelGrid[l],
                                                                         itsLevelOffset[l - 1], itsLevelGrid[l - 1]);
@@ -328,7 +370,7 @@ void OctreeBuilder::mergeLevels()
     itsNodeParent.ensure(sizeof(int32_t) * m);
     itsNodeChild.ensure(sizeof(int32_t) * 8 * m);
     itsNodeFlags.ensure(m);
-    itsNodeFill.ensure(sizeof(uint32_t) * m);
+    itsNodeSorted.ensure(sizeof(uint32_t) * m);
     itsNodeAux.ensure(sizeof(uint32_t) * (m + 1));
     itsNodeStage.ensure(sizeof(uint32_t) * m);
     itsNodeSub.ensure(sizeof(uint32_t) * m);
@@ -346,8 +388,7 @@ void OctreeBuilder::mergeLevels()
     kNodeLinks<<<gridFor(m), BLOCK, 0, itsStream>>>(
             itsM, lp, itsNodeKey.as<uint32_t>(), itsNodeCount.as<uint32_t>(), itsDenseToSparse.as<int32_t>(),
             itsNodePoints.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsNodeParent.as<int32_t>(),
-            itsNodeChild.as<int32_t>(),
-            itsNodeFlags.as<uint8_t>(), itsNodeFill.as<uint32_t>(), itsNodeAux.as<uint32_t>(),
+            itsNodeChild.as<int32_t>(), itsNodeFlags.as<uint8_t>(), itsNodeAux.as<uint32_t>(),
             itsNodeSub.as<uint32_t>());
     OC_LAUNCH_CHECK();
 
@@ -373,11 +414,24 @@ void OctreeBuilder::distributePoints()
                itsProc.chunkingGrid};
     // octree_chunking.cu:108-112
     double sx = 1.0 / itsCloud.scale[0], sy = 1.0 / itsCloud.scale[1], sz = 1.0 / itsCloud.scale[2];
+    LevelParams lp{};
+    for (uint32_t l = 0; l <= itsDepth; ++l)
+    {
+        lp.offset[l] = itsLevelOffset[l];
+        lp.grid[l]   = itsLevelGrid[l];
+    }
+    lp.depth     = itsDepth;
+    lp.threshold = itsProc.mergingThreshold;
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_DIST_BEGIN], itsStream));
-    kDistribute<T><<<gridFor(itsCloud.pointAmount, 16384), BLOCK, 0, itsStream>>>(
-            itsCloudDevice, itsCloud.pointAmount, itsCloud.dataStride, g, itsCount.as<uint32_t>(),
-            itsNodeFill.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),
-            itsPts.as<T>(), sx, sy, sz);
+    kSortedStart<<<gridFor(itsM), BLOCK, 0, itsStream>>>(itsM, lp, itsNodeKey.as<uint32_t>(),
+                                                         itsNodeCount.as<uint32_t>(), itsNodeFlags.as<uint8_t>(),
+                                                         itsNodeParent.as<int32_t>(), itsNodeChild.as<int32_t>(),
+                                                         itsNodeSorted.as<uint32_t>());
+    OC_LAUNCH_CHECK();
+    kDistributeSorted<T><<<gridFor(itsCloud.pointAmount, 16384), BLOCK, 0, itsStream>>>(
+            itsSorted.as<SortedPoint<T>>(), itsCloud.pointAmount, g, itsCount.as<uint32_t>(),
+            itsNodeSorted.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),
+            sx, sy, sz);
     OC_LAUNCH_CHECK();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_DIST_END], itsStream));
 }
@@ -421,8 +475,16 @@ void OctreeBuilder::subsampleLevels()
     const bool packed   = itsColorMode == COLOR_INTRA || itsColorMode == COLOR_INTER;
     const bool weighted = itsColorMode == COLOR_INTER_WEIGHTED;
     uint64_t cursor     = N;
-    PointStore<T> ps{itsLut.as<uint32_t>(),   itsOut.as<OutRecord>(),   itsPts.as<T>(), itsStLut.as<uint32_t>(),
-                     itsStOut.as<OutRecord>(), itsStPts.as<T>(),         N};
+    PointStore<T> ps{};
+    ps.sorted = itsSorted.as<SortedPoint<T>>();
+    ps.stLut  = itsStLut.as<uint32_t>();
+    ps.stOut  = itsStOut.as<OutRecord>();
+    ps.stPts  = itsStPts.as<T>();
+    ps.N      = N;
+    ps.cubic  = Gridding{itsCloud.bbMin[0], itsCloud.bbMin[1], itsCloud.bbMin[2], cubicSize, itsProc.chunkingGrid};
+    ps.sx     = 1.0 / itsCloud.scale[0];
+    ps.sy     = 1.0 / itsCloud.scale[1];
+    ps.sz     = 1.0 / itsCloud.scale[2];
 
     SubsampleParams sp{};
     for (int a = 0; a < 3; ++a)
@@ -462,7 +524,7 @@ void OctreeBuilder::subsampleLevels()
         itsNodeGrid.ensure(sizeof(NodeGrid) * K);
         kSegments<<<gridFor(K), BLOCK, 0, itsStream>>>(
                 K, sp, itsInternalList.as<uint32_t>(), itsNodeKey.as<uint32_t>(), itsNodeFlags.as<uint8_t>(),
-                itsNodeChild.as<int32_t>(), itsNodePoints.as<uint32_t>(), itsNodeData.as<uint64_t>(),
+                itsNodeChild.as<int32_t>(), itsNodePoints.as<uint32_t>(), itsNodeSorted.as<uint32_t>(),
                 itsNodeStage.as<uint32_t>(), itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(),
                 itsNodeGrid.as<NodeGrid>());
         OC_LAUNCH_CHECK();

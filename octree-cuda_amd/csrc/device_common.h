@@ -1,5 +1,5 @@
 // Device-side helpers shared by the builder's kernels: record decoding, the cell mapping, packed
-// colours, the explicit-random ranking and the per-node open-addressing tables.
+// colours, quantisation and the explicit-random ranking.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -92,6 +92,22 @@ __host__ __device__ __forceinline__ uint32_t nodeSalt(uint64_t seed, uint32_t no
 {
     return static_cast<uint32_t>(
             splitmix64Mix(seed + 0x9E3779B97F4A7C15ULL * (static_cast<uint64_t>(nodeKey) + 1)) >> 32);
+}
+
+// Output record of a point (point_distributing.cuh:62-74): int32(floor((p - cubicMin) * (1 / scale)))
+// per axis with the reciprocal taken on the host (octree_chunking.cu:108-112); rgb packs r | g << 8
+// | b << 16.
+__device__ __forceinline__ OutRecord quantizePoint(double x, double y, double z, uint32_t rgb, const Gridding& g,
+                                                   double sx, double sy, double sz)
+{
+    OutRecord o;
+    o.x = static_cast<int32_t>(floor((x - g.minX) * sx));
+    o.y = static_cast<int32_t>(floor((y - g.minY) * sy));
+    o.z = static_cast<int32_t>(floor((z - g.minZ) * sz));
+    o.r = static_cast<uint16_t>(rgb & 0xFF);
+    o.g = static_cast<uint16_t>((rgb >> 8) & 0xFF);
+    o.b = static_cast<uint16_t>((rgb >> 16) & 0xFF);
+    return o;
 }
 
 __device__ __forceinline__ void storeRecord(OutRecord* dst, const OutRecord& r)

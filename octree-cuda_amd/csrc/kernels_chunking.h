@@ -1,4 +1,5 @@
-// Kernels of the counting, merging and distribution phases.
+// Kernels of the merging phase: count pyramid, canonical sparse ids, node links and flags, leaf
+// data blocks and the level-0 cell -> leaf map.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,27 +46,6 @@ __global__ __launch_bounds__(BLOCK) void kFill64(uint64_t* p, uint64_t n, uint64
 {
     for (uint64_t i = globalThread(); i < n; i += globalStride())
         p[i] = v;
-}
-
-// K1: kernelPointCounting (point_counting.cuh:30-53) - dense level-0 histogram. Sparse ids are
-// assigned afterwards by a scan (canonical order), so no global id counter is touched here.
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void kCount(const uint8_t* __restrict__ cloud, uint32_t n, uint32_t stride,
-                                                 Gridding g, uint32_t* __restrict__ count, uint32_t* __restrict__ err)
-{
-    for (uint64_t i = globalThread(); i < n; i += globalStride())
-    {
-        double x, y, z;
-        loadXYZ<T>(cloud + i * stride, x, y, z);
-        uint64_t ix, iy, iz;
-        gridCoords(x, y, z, g, ix, iy, iz);
-        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
-        {
-            atomicOr(err, ERR_OUTSIDE);
-            continue;
-        }
-        atomicAdd(count + static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize), 1u);
-    }
 }
 
 // K2 (counts only): kernelPropagatePointCounts (point_count_propagation.cuh:34-86). Child slot k
@@ -147,7 +127,7 @@ __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, 
                                                      uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
                                                      int32_t* __restrict__ parent,
                                                      int32_t* __restrict__ children, uint8_t* __restrict__ flags,
-                                                     uint32_t* __restrict__ fill, uint32_t* __restrict__ blockSize,
+                                                     uint32_t* __restrict__ blockSize,
                                                      uint32_t* __restrict__ subtree)
 {
     for (uint64_t i = globalThread(); i < m; i += globalStride())
@@ -196,7 +176,6 @@ __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, 
         dataIdx[id]    = 0;
         parent[id]     = p;
         flags[id]      = static_cast<uint8_t>((finished ? NODE_FINISHED : 0) | (internal ? NODE_INTERNAL : 0));
-        fill[id]       = 0;
         blockSize[id]  = blk;
         subtree[id]    = 0;
     }
@@ -257,44 +236,6 @@ __global__ __launch_bounds__(BLOCK) void kCellLeaf(uint32_t level0Nodes, const u
         while (!(flags[s] & NODE_FINISHED))
             s = parent[s];
         count[nodeKey[i]] = static_cast<uint32_t>(s);
-    }
-}
-
-// K5: kernelDistributePoints (point_distributing.cuh:31-76). Slot within the leaf by atomic
-// (order unconstrained, A5); writes the point index, the quantised record and the coordinates.
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void kDistribute(const uint8_t* __restrict__ cloud, uint32_t n, uint32_t stride,
-                                                      Gridding g, const uint32_t* __restrict__ cellLeaf,
-                                                      uint32_t* __restrict__ fill,
-                                                      const uint64_t* __restrict__ dataIdx,
-                                                      uint32_t* __restrict__ lut, OutRecord* __restrict__ out,
-                                                      T* __restrict__ pts, double sx, double sy, double sz)
-{
-    for (uint64_t i = globalThread(); i < n; i += globalStride())
-    {
-        const uint8_t* rec = cloud + i * stride;
-        T v[3];
-        __builtin_memcpy(v, rec, sizeof(v));
-        double x = v[0], y = v[1], z = v[2];
-        uint64_t ix, iy, iz;
-        gridCoords(x, y, z, g, ix, iy, iz);
-        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
-            continue;  // reported by the counting pass
-        uint32_t cell = static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
-        uint32_t leaf = cellLeaf[cell];
-        uint32_t slot = atomicAdd(fill + leaf, 1u);
-        uint32_t pos  = static_cast<uint32_t>(dataIdx[leaf]) + slot;
-        lut[pos]      = static_cast<uint32_t>(i);
-        OutRecord o;
-        o.x = static_cast<int32_t>(floor((x - g.minX) * sx));
-        o.y = static_cast<int32_t>(floor((y - g.minY) * sy));
-        o.z = static_cast<int32_t>(floor((z - g.minZ) * sz));
-        const uint8_t* col = rec + 3 * sizeof(T);
-        o.r = col[0];
-        o.g = col[1];
-        o.b = col[2];
-        storeRecord(out + pos, o);
-        __builtin_memcpy(pts + 3 * static_cast<uint64_t>(pos), v, sizeof(v));
     }
 }
 

@@ -1,0 +1,347 @@
+// Counting and distribution as a two-pass Morton bucket sort of the cloud.
+//
+// The cloud arrives in arbitrary spatial order, so counting it straight into the level-0 grid and
+// scattering every point to its leaf costs one random global atomic and a handful of random
+// partial-line stores per point. Here the points are sorted by the Morton code of their level-0
+// cell instead (child slot order: x bit 4, y bit 2, z bit 1, as point_count_propagation.cuh:62-70),
+// in two passes that only use LDS atomics per point:
+//   kBucketHist     per-workgroup LDS histogram of coarse buckets (level-lc cells) -> bucket sizes
+//   kBucketScatter  each workgroup ranks its chunk of 16K points per bucket in LDS, reserves one
+//                   range per bucket with a single global atomic and writes 32-byte sorted-point
+//                   records (coordinates, point index, colour) plus the 3*lc-bit fine code
+//   kFineSort       one workgroup per bucket: LDS histogram over the bucket's (2^lc)^3 level-0
+//                   cells, exclusive scan, dense counts of levels 0..lc of the count pyramid,
+//                   then a scatter into full Morton order
+// In Morton order every octree node is one contiguous range of the sorted cloud, so after the
+// merge a leaf's points are the range [sortedStart, sortedStart + count) and the distribution is
+// a coalesced copy of those ranges into the leaf data blocks (kDistributeSorted).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.h"
+#include "kernels_chunking.h"
+#include "scan.h"
+#include "types.h"
+
+namespace oc {
+
+constexpr uint32_t MAX_COARSE_LEVEL = 5;                               // (2^5)^3 level-0 cells per bucket
+constexpr uint32_t MAX_FINE_CELLS   = 1u << (3 * MAX_COARSE_LEVEL);    // 32768
+constexpr uint32_t MAX_BUCKETS      = 4096;                            // (512 >> 5)^3
+constexpr int SORT_THREADS          = 1024;
+constexpr int SORT_WAVES            = SORT_THREADS / 64;
+constexpr int SCATTER_ITEMS         = 16;
+constexpr uint32_t SCATTER_CHUNK    = SORT_THREADS * SCATTER_ITEMS;    // points per scatter workgroup
+constexpr int HIST_THREADS          = 512;
+
+// A point of the sorted cloud: the original coordinates, its index in the input cloud and its
+// colour (r | g << 8 | b << 16).
+template <typename T>
+struct alignas(sizeof(T) == 8 ? 16 : 4) SortedPoint
+{
+    T x, y, z;
+    uint32_t idx;
+    uint32_t rgb;
+};
+static_assert(sizeof(SortedPoint<double>) == 32, "double sorted point is 32 bytes");
+static_assert(sizeof(SortedPoint<float>) == 20, "float sorted point is 20 bytes");
+
+// Bit interleaving for coordinates below 2^10 (part1by2 / compact1by2).
+__host__ __device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    v &= 0x000003FFu;
+    v = (v ^ (v << 16)) & 0xFF0000FFu;
+    v = (v ^ (v << 8)) & 0x0300F00Fu;
+    v = (v ^ (v << 4)) & 0x030C30C3u;
+    v = (v ^ (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t compact3(uint32_t v)
+{
+    v &= 0x09249249u;
+    v = (v ^ (v >> 2)) & 0x030C30C3u;
+    v = (v ^ (v >> 4)) & 0x0300F00Fu;
+    v = (v ^ (v >> 8)) & 0xFF0000FFu;
+    v = (v ^ (v >> 16)) & 0x000003FFu;
+    return v;
+}
+
+// Morton code whose 3-bit groups are child slots (x -> 4, y -> 2, z -> 1).
+__host__ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+
+// Exclusive scan over a 1024-thread workgroup; `total` receives the sum. Contains barriers.
+__device__ __forceinline__ uint32_t sortBlockExclusiveScan(uint32_t v, uint32_t& total, uint32_t* lds)
+{
+    const uint32_t inc = waveInclusiveScan(v);
+    const int w        = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63)
+        lds[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, sum = 0;
+#pragma unroll
+    for (int i = 0; i < SORT_WAVES; ++i)
+    {
+        const uint32_t s = lds[i];
+        off += (i < w) ? s : 0u;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return inc - v + off;
+}
+
+// Level-0 Morton code of a point, or ~0u when it lies outside the grid.
+__device__ __forceinline__ uint32_t pointMorton(double x, double y, double z, const Gridding& g)
+{
+    uint64_t ix, iy, iz;
+    gridCoords(x, y, z, g, ix, iy, iz);
+    if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
+        return ~0u;
+    return morton3(static_cast<uint32_t>(ix), static_cast<uint32_t>(iy), static_cast<uint32_t>(iz));
+}
+
+template <typename T>
+__global__ __launch_bounds__(HIST_THREADS) void kBucketHist(const uint8_t* __restrict__ cloud, uint32_t n,
+                                                             uint32_t stride, Gridding g, uint32_t fineBits,
+                                                             uint32_t buckets, uint32_t* __restrict__ bucketCount,
+                                                             uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t h[MAX_BUCKETS];
+    for (uint32_t b = threadIdx.x; b < buckets; b += HIST_THREADS)
+        h[b] = 0;
+    __syncthreads();
+    bool outside = false;
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        double x, y, z;
+        loadXYZ<T>(cloud + i * stride, x, y, z);
+        const uint32_t m = pointMorton(x, y, z, g);
+        if (m == ~0u)
+        {
+            outside = true;
+            continue;
+        }
+        atomicAdd(h + (m >> fineBits), 1u);
+    }
+    if (outside)
+        atomicOr(err, ERR_OUTSIDE);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < buckets; b += HIST_THREADS)
+        if (h[b])
+            atomicAdd(bucketCount + b, h[b]);
+}
+
+// bucket sizes -> bucket starts (exclusive) and the scatter cursors
+struct BucketStartOut
+{
+    uint32_t* start;
+    uint32_t* cursor;
+    __device__ void operator()(uint64_t i, uint32_t prefix, uint32_t) const
+    {
+        start[i]  = prefix;
+        cursor[i] = prefix;
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
+                                                                uint32_t stride, Gridding g, uint32_t fineBits,
+                                                                uint32_t buckets, uint32_t* __restrict__ cursor,
+                                                                SortedPoint<T>* __restrict__ dst,
+                                                                uint16_t* __restrict__ fineCode)
+{
+    __shared__ uint32_t h[MAX_BUCKETS];
+    for (uint32_t b = threadIdx.x; b < buckets; b += SORT_THREADS)
+        h[b] = 0;
+    __syncthreads();
+
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * SCATTER_CHUNK;
+    uint32_t code[SCATTER_ITEMS], rank[SCATTER_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SCATTER_ITEMS; ++j)
+    {
+        const uint64_t i = base + static_cast<uint64_t>(j) * SORT_THREADS + threadIdx.x;
+        code[j]          = ~0u;
+        rank[j]          = 0;
+        if (i < n)
+        {
+            double x, y, z;
+            loadXYZ<T>(cloud + i * stride, x, y, z);
+            code[j] = pointMorton(x, y, z, g);
+            if (code[j] != ~0u)
+                rank[j] = atomicAdd(h + (code[j] >> fineBits), 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < buckets; b += SORT_THREADS)
+    {
+        const uint32_t c = h[b];
+        if (c)
+            h[b] = atomicAdd(cursor + b, c);
+    }
+    __syncthreads();
+    const uint32_t fineMask = (1u << fineBits) - 1u;
+#pragma unroll
+    for (int j = 0; j < SCATTER_ITEMS; ++j)
+    {
+        if (code[j] == ~0u)
+            continue;
+        const uint64_t i   = base + static_cast<uint64_t>(j) * SORT_THREADS + threadIdx.x;
+        const uint8_t* rec = cloud + i * stride;
+        SortedPoint<T> p;
+        __builtin_memcpy(&p.x, rec, 3 * sizeof(T));
+        const uint8_t* col = rec + 3 * sizeof(T);
+        p.idx              = static_cast<uint32_t>(i);
+        p.rgb              = static_cast<uint32_t>(col[0]) | (static_cast<uint32_t>(col[1]) << 8) |
+                static_cast<uint32_t>(col[2]) << 16;
+        const uint32_t d = h[code[j] >> fineBits] + rank[j];
+        if (d >= n)
+            continue;  // cannot happen: both passes compute the same codes
+        dst[d]           = p;
+        fineCode[d]      = static_cast<uint16_t>(code[j] & fineMask);
+    }
+}
+
+struct FineSortParams
+{
+    uint32_t lc;                   // coarse level: a bucket is one level-lc cell
+    uint32_t bucketGrid;           // buckets per axis (chunking grid >> lc)
+    uint32_t levelOffset[MAX_LEVELS];
+    uint32_t levelGrid[MAX_LEVELS];
+};
+
+template <typename T>
+__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const SortedPoint<T>* __restrict__ src,
+                                                           const uint16_t* __restrict__ fineCode,
+                                                           const uint32_t* __restrict__ bucketStart, FineSortParams fp,
+                                                           uint32_t* __restrict__ count,
+                                                           SortedPoint<T>* __restrict__ dst)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
+    __shared__ uint32_t scratch[SORT_WAVES];
+    const uint32_t b     = blockIdx.x;
+    const uint32_t cells = 1u << (3 * fp.lc);
+    const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
+
+    for (uint32_t c = threadIdx.x; c <= cells; c += SORT_THREADS)
+        h[c] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
+        atomicAdd(h + fineCode[i], 1u);
+    __syncthreads();
+
+    uint32_t carry = 0;
+    for (uint32_t r = 0; r < cells; r += SORT_THREADS)
+    {
+        const uint32_t c = r + threadIdx.x;
+        const uint32_t v = c < cells ? h[c] : 0u;
+        uint32_t total;
+        const uint32_t ex = sortBlockExclusiveScan(v, total, scratch);
+        if (c < cells)
+            h[c] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        h[cells] = carry;
+    __syncthreads();
+
+    // dense counts of this bucket's cells on levels 0..lc (kernelPropagatePointCounts,
+    // point_count_propagation.cuh:34-86, as differences of the Morton prefix)
+    const uint32_t bx = compact3(b >> 2), by = compact3(b >> 1), bz = compact3(b);
+    for (uint32_t k = 0; k <= fp.lc; ++k)
+    {
+        const uint32_t side   = 1u << (fp.lc - k);
+        const uint32_t n3     = side * side * side;
+        const uint64_t gk     = fp.levelGrid[k];
+        const uint64_t offset = fp.levelOffset[k];
+        for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
+        {
+            const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
+            const uint32_t m  = morton3(lx, ly, lz);
+            const uint32_t v  = h[(m + 1) << (3 * k)] - h[m << (3 * k)];
+            const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
+                           gz = static_cast<uint64_t>(bz) * side + lz;
+            count[offset + gx + gy * gk + gz * gk * gk] = v;
+        }
+    }
+    __syncthreads();
+
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
+    {
+        const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
+        dst[lo + slot]      = src[i];
+    }
+}
+
+// Start of every leaf's range in the sorted cloud: the points of all nodes that precede it in
+// Morton order, i.e. of the lower-slot siblings of the leaf and of each of its ancestors.
+__global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
+                                                       const uint32_t* __restrict__ nodeCount,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const int32_t* __restrict__ parent,
+                                                       const int32_t* __restrict__ children,
+                                                       uint32_t* __restrict__ sortedStart)
+{
+    for (uint64_t i = globalThread(); i < m; i += globalStride())
+    {
+        const uint32_t id = static_cast<uint32_t>(i);
+        if ((flags[id] & (NODE_FINISHED | NODE_INTERNAL)) != NODE_FINISHED)
+            continue;
+        uint32_t start = 0;
+        int32_t cur    = static_cast<int32_t>(id);
+        int32_t p      = parent[cur];
+        while (p >= 0)
+        {
+            const uint32_t key = nodeKey[cur];
+            const uint32_t l   = levelOfKey(key, lp);
+            const uint32_t g   = lp.grid[l];
+            const uint32_t lin = key - lp.offset[l];
+            const uint32_t x = lin % g, y = (lin / g) % g, z = lin / (g * g);
+            const int slot     = static_cast<int>(((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
+            for (int k = 0; k < slot; ++k)
+            {
+                const int32_t c = children[8 * static_cast<uint64_t>(p) + k];
+                if (c >= 0)
+                    start += nodeCount[c];
+            }
+            cur = p;
+            p   = parent[cur];
+        }
+        sortedStart[id] = start;
+    }
+}
+
+// K5: kernelDistributePoints (point_distributing.cuh:31-76) over the sorted cloud. The leaf of a
+// point is the one of its level-0 cell (cellLeaf); its slot is its offset in the leaf's sorted
+// range, so consecutive threads write consecutive records of one leaf block.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kDistributeSorted(const SortedPoint<T>* __restrict__ sorted, uint32_t n,
+                                                            Gridding g, const uint32_t* __restrict__ cellLeaf,
+                                                            const uint32_t* __restrict__ sortedStart,
+                                                            const uint64_t* __restrict__ dataIdx,
+                                                            uint32_t* __restrict__ lut, OutRecord* __restrict__ out,
+                                                            double sx, double sy, double sz)
+{
+    for (uint64_t s = globalThread(); s < n; s += globalStride())
+    {
+        const SortedPoint<T> p = sorted[s];
+        uint64_t ix, iy, iz;
+        gridCoords(static_cast<double>(p.x), static_cast<double>(p.y), static_cast<double>(p.z), g, ix, iy, iz);
+        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
+            continue;  // only in-grid points are sorted
+        const uint32_t cell = static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
+        const uint32_t leaf = cellLeaf[cell];
+        const uint64_t pos  = dataIdx[leaf] + (s - sortedStart[leaf]);
+        lut[pos]            = p.idx;
+        storeRecord(out + pos, quantizePoint(p.x, p.y, p.z, p.rgb, g, sx, sy, sz));
+    }
+}
+
+}  // namespace oc

@@ -19,33 +19,55 @@
 
 #include "device_common.h"
 #include "kernels_chunking.h"
+#include "kernels_sort.h"
 #include "types.h"
 
 namespace oc {
 
 constexpr uint32_t TILE = 256;  // inputs per workgroup in the per-input kernels
 
-// The points a subsampling level reads: leaf points at positions [0, N) (in their final place in
-// the output) and already subsampled points of lower levels at positions [N, N + I) (staging, in
-// level order until the final DFS post-order placement).
+// The points a subsampling level reads: leaf points at positions [0, N) of the Morton-sorted
+// cloud (a leaf's points are one contiguous range there) and already subsampled points of lower
+// levels at positions [N, N + I) (staging, in level order until the final DFS post-order
+// placement). A leaf point's output record is derived from its sorted record exactly as the
+// distribution writes it.
 template <typename T>
 struct PointStore
 {
-    uint32_t* lut;
-    OutRecord* out;
-    T* pts;
+    const SortedPoint<T>* sorted;
     uint32_t* stLut;
     OutRecord* stOut;
     T* stPts;
     uint32_t N;
-    __device__ __forceinline__ uint32_t index(uint32_t pos) const { return pos < N ? lut[pos] : stLut[pos - N]; }
+    Gridding cubic;
+    double sx, sy, sz;
+    __device__ __forceinline__ uint32_t index(uint32_t pos) const
+    {
+        return pos < N ? sorted[pos].idx : stLut[pos - N];
+    }
     __device__ __forceinline__ OutRecord record(uint32_t pos) const
     {
-        return loadRecord(pos < N ? out + pos : stOut + (pos - N));
+        if (pos < N)
+        {
+            const SortedPoint<T>& p = sorted[pos];
+            return quantizePoint(p.x, p.y, p.z, p.rgb, cubic, sx, sy, sz);
+        }
+        return loadRecord(stOut + (pos - N));
     }
-    __device__ __forceinline__ const T* xyz(uint32_t pos) const
+    __device__ __forceinline__ void xyz(uint32_t pos, T& x, T& y, T& z) const
     {
-        return pos < N ? pts + 3 * static_cast<uint64_t>(pos) : stPts + 3 * static_cast<uint64_t>(pos - N);
+        if (pos < N)
+        {
+            const SortedPoint<T>& p = sorted[pos];
+            x                        = p.x;
+            y                        = p.y;
+            z                        = p.z;
+            return;
+        }
+        const T* q = stPts + 3 * static_cast<uint64_t>(pos - N);
+        x          = q[0];
+        y          = q[1];
+        z          = q[2];
     }
 };
 
@@ -81,8 +103,8 @@ struct InternalOut
     }
 };
 
-// Per internal node k of the level: its eight input segments (children in slot order, leaf data
-// or the child's staged subsample) and its gridding. The node minimum is accumulated top-down
+// Per internal node k of the level: its eight input segments (children in slot order, the leaf's
+// range of the sorted cloud or the child's staged subsample) and its gridding. The node minimum is accumulated top-down
 // exactly like the reference's DFS (calculateChildMinBB, tools.cu:29-61; rp_subsampling.cu:16-60):
 // x = min.x; for every step down whose slot has the x bit, x = x + side/2 of that step; the grid
 // extent is bbSize = (min.x + side) - min.x.
@@ -92,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsamplePara
                                                     const uint8_t* __restrict__ flags,
                                                     const int32_t* __restrict__ children,
                                                     const uint32_t* __restrict__ pointCount,
-                                                    const uint64_t* __restrict__ dataIdx,
+                                                    const uint32_t* __restrict__ sortedStart,
                                                     const uint32_t* __restrict__ stage,
                                                     uint32_t* __restrict__ segCount, uint32_t* __restrict__ segSrc,
                                                     NodeGrid* __restrict__ nodeGrid)
@@ -107,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsamplePara
             if (c >= 0)
             {
                 cnt = pointCount[c];
-                src = (flags[c] & NODE_INTERNAL) ? stage[c] : static_cast<uint32_t>(dataIdx[c]);
+                src = (flags[c] & NODE_INTERNAL) ? stage[c] : sortedStart[c];
             }
             segCount[8 * k + s] = cnt;
             segSrc[8 * k + s]   = src;
@@ -213,9 +235,10 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
     if (!in.valid)
         return;
     const NodeGrid ng = nodeGrid[in.k];
-    const T* p        = ps.xyz(in.pos);
+    T px, py, pz;
+    ps.xyz(in.pos, px, py, pz);
     uint64_t ix, iy, iz;
-    gridCoords(static_cast<double>(p[0]), static_cast<double>(p[1]), static_cast<double>(p[2]), ng.grid, ix, iy, iz);
+    gridCoords(static_cast<double>(px), static_cast<double>(py), static_cast<double>(pz), ng.grid, ix, iy, iz);
     const uint32_t gs = ng.grid.gridSize;
     if (ix >= gs || iy >= gs || iz >= gs)
     {
@@ -350,13 +373,10 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
     if (!in.valid)
         return;
-    const T* p         = ps.xyz(in.pos);
     OutRecord r        = ps.record(in.pos);
     const uint32_t idx = ps.index(in.pos);
     Payload<T> pl;
-    pl.x    = p[0];
-    pl.y    = p[1];
-    pl.z    = p[2];
+    ps.xyz(in.pos, pl.x, pl.y, pl.z);
     pl.rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
     pl.pos  = in.pos;
     pl.r    = r.r;
@@ -501,11 +521,8 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
         const uint32_t st = cursor - ps.N + c;  // staging index
         storeRecord(ps.stOut + st, rec);
         ps.stLut[st] = ps.index(wpos);
-        const T* src = ps.xyz(wpos);
         T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
-        d[0]         = src[0];
-        d[1]         = src[1];
-        d[2]         = src[2];
+        ps.xyz(wpos, d[0], d[1], d[2]);
         stageNode[st] = ng.id;
     }
 }
