@@ -544,7 +544,7 @@ void OctreeBuilder::subsampleLevels()
         {
             itsCompactOf.ensure(sizeof(uint32_t) * inputs);
             itsRankOf.ensure(sizeof(uint32_t) * inputs);
-            itsPayload.ensure(sizeof(Payload<T>) * inputs);
+            itsPayload.ensure(sizeof(Payload<T, true>) * inputs);
         }
         kTileList<<<gridFor(segments), BLOCK, 0, itsStream>>>(segments, itsSegCount.as<uint32_t>(),
                                                               itsSegTile.as<uint32_t>(), itsTileList.as<uint32_t>());
@@ -644,33 +644,52 @@ void OctreeBuilder::subsampleLevels()
                                 tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
-                                itsCellStart.as<uint32_t>(), ps, itsPayload.as<Payload<T>>());
+                                itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
+                                itsPayload.as<Payload<T, true>>());
                     else
                         kScatterPayload<T, false><<<blocks, TILE, 0, itsStream>>>(
                                 tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
-                                itsCellStart.as<uint32_t>(), ps, itsPayload.as<Payload<T>>());
+                                itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
+                                itsPayload.as<Payload<T, false>>());
                     OC_LAUNCH_CHECK();
                 }
             }
 
             OC_HIP_CHECK(hipEventRecord(ev[2], itsStream));
-            switch (itsColorMode)
+            if (weighted)
             {
-#define OC_EMIT(M)                                                                                                     \
-    kEmit<T, M><<<gridFor(cells), BLOCK, 0, itsStream>>>(                                                              \
-            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap, prefix,      \
-            itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsWin.as<uint64_t>(), color,                      \
-            weighted ? itsCellStart.as<uint32_t>() : nullptr, weighted ? itsPayload.as<Payload<T>>() : nullptr, ps,   \
-            itsStageNode.as<uint32_t>())
-            case COLOR_NONE: OC_EMIT(COLOR_NONE); break;
-            case COLOR_INTRA: OC_EMIT(COLOR_INTRA); break;
-            case COLOR_INTER: OC_EMIT(COLOR_INTER); break;
-            default: OC_EMIT(COLOR_INTER_WEIGHTED); break;
-#undef OC_EMIT
+                const uint32_t emitBlocks = static_cast<uint32_t>((cells + BLOCK - 1) / BLOCK);
+                if (itsProc.useRandomSubsampling)
+                    kEmitWeighted<T, true><<<emitBlocks, BLOCK, 0, itsStream>>>(
+                            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
+                            prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
+                            itsPayload.as<Payload<T, true>>(), ps, itsStageNode.as<uint32_t>());
+                else
+                    kEmitWeighted<T, false><<<emitBlocks, BLOCK, 0, itsStream>>>(
+                            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
+                            prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
+                            itsPayload.as<Payload<T, false>>(), ps, itsStageNode.as<uint32_t>());
+                OC_LAUNCH_CHECK();
             }
-            OC_LAUNCH_CHECK();
+            else
+            {
+                switch (itsColorMode)
+                {
+#define OC_EMIT(M)                                                                                                     \
+    kEmit<T, M><<<gridFor(cells), BLOCK, 0, itsStream>>>(cells, k0, wordsPerNode, static_cast<uint32_t>(cursor),      \
+                                                         itsNodeGrid.as<NodeGrid>(), bitmap, prefix,                  \
+                                                         itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(),       \
+                                                         itsWin.as<uint64_t>(), itsColor.as<uint64_t>(), ps,          \
+                                                         itsStageNode.as<uint32_t>())
+                case COLOR_NONE: OC_EMIT(COLOR_NONE); break;
+                case COLOR_INTRA: OC_EMIT(COLOR_INTRA); break;
+                default: OC_EMIT(COLOR_INTER); break;
+#undef OC_EMIT
+                }
+                OC_LAUNCH_CHECK();
+            }
             OC_HIP_CHECK(hipEventRecord(ev[3], itsStream));
             cursor += cells;
         }

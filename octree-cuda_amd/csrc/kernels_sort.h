@@ -97,6 +97,29 @@ __device__ __forceinline__ uint32_t sortBlockExclusiveScan(uint32_t v, uint32_t&
     return inc - v + off;
 }
 
+// Exclusive scan over a workgroup of THREADS threads (a multiple of 64); `total` receives the sum.
+// Contains barriers.
+template <int THREADS>
+__device__ __forceinline__ uint32_t sortBlockExclusiveScanN(uint32_t v, uint32_t& total, uint32_t* lds)
+{
+    const uint32_t inc = waveInclusiveScan(v);
+    const int w        = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63)
+        lds[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, sum = 0;
+#pragma unroll
+    for (int i = 0; i < THREADS / 64; ++i)
+    {
+        const uint32_t s = lds[i];
+        off += (i < w) ? s : 0u;
+        sum += s;
+    }
+    __syncthreads();
+    total = sum;
+    return inc - v + off;
+}
+
 // Level-0 Morton code of a point, or ~0u when it lies outside the grid.
 __device__ __forceinline__ uint32_t pointMorton(double x, double y, double z, const Gridding& g)
 {

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
     if (ix >= gs || iy >= gs || iz >= gs)
     {
         atomicOr(err, ERR_OUTSIDE);
-        cellOf[in.q] = 0;
+        cellOf[in.q] = ~0u;  // skipped by the later passes; the build reports the error
         return;
     }
     const uint32_t cell = static_cast<uint32_t>(ix + iy * gs + iz * gs * gs);
@@ -321,9 +321,9 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
     if (!in.valid)
         return;
     const uint32_t cell = cellOf[in.q];
-    const uint64_t word = (in.k - nodeBase) * wordsPerNode + (cell >> 5);
-    if (!((bitmap[word] >> (cell & 31)) & 1u))
+    if (cell == ~0u)
         return;  // the point lay outside its node's grid (reported by kMark)
+    const uint64_t word = (in.k - nodeBase) * wordsPerNode + (cell >> 5);
     const uint32_t c = cellRank(bitmap, prefix, word, cell & 31);
     // every point of the cell stores the same key and node
     __hip_atomic_store(cellKey + c, cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -348,14 +348,24 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
 }
 
 // Point payload of the weighted filter, stored in cell order so that a cell's points are one
-// contiguous run: coordinates in the cloud's type, the winner rank and input position, and the
-// input record colour.
-template <typename T>
-struct alignas(8) Payload
+// contiguous run: coordinates in the cloud's type, the source point index and colour (r | g << 8 |
+// b << 16; weighted averages of 8-bit colours stay below 256), and for the explicit method the
+// point's rank. The winner of a cell is its point of minimum rank (the index itself for the
+// implicit method; both are bijective, so there are no ties).
+template <typename T, bool RANDOM>
+struct alignas(sizeof(T) == 8 ? 16 : 4) Payload
 {
     T x, y, z;
-    uint32_t rank, pos;
-    uint16_t r, g, b, pad;
+    uint32_t idx, rgb;
+    __device__ __forceinline__ uint32_t rank() const { return idx; }
+};
+
+template <typename T>
+struct alignas(8) Payload<T, true>
+{
+    T x, y, z;
+    uint32_t idx, rgb, rnk;
+    __device__ __forceinline__ uint32_t rank() const { return rnk; }
 };
 
 template <typename T, bool RANDOM>
@@ -367,35 +377,30 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
                                                          const NodeGrid* __restrict__ nodeGrid,
                                                          const uint32_t* __restrict__ compactOf,
                                                          const uint32_t* __restrict__ rankOf,
-                                                         const uint32_t* __restrict__ cellStart, PointStore<T> ps,
-                                                         Payload<T>* __restrict__ payload)
+                                                         const uint32_t* __restrict__ cellStart,
+                                                         const uint32_t* __restrict__ cellOf, PointStore<T> ps,
+                                                         Payload<T, RANDOM>* __restrict__ payload)
 {
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
-    if (!in.valid)
+    if (!in.valid || cellOf[in.q] == ~0u)
         return;
-    OutRecord r        = ps.record(in.pos);
-    const uint32_t idx = ps.index(in.pos);
-    Payload<T> pl;
+    const OutRecord r = ps.record(in.pos);
+    Payload<T, RANDOM> pl;
     ps.xyz(in.pos, pl.x, pl.y, pl.z);
-    pl.rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
-    pl.pos  = in.pos;
-    pl.r    = r.r;
-    pl.g    = r.g;
-    pl.b    = r.b;
-    pl.pad  = 0;
+    pl.idx = ps.index(in.pos);
+    pl.rgb = static_cast<uint32_t>(r.r) | (static_cast<uint32_t>(r.g) << 8) | (static_cast<uint32_t>(r.b) << 16);
+    if constexpr (RANDOM)
+        pl.rnk = fmix32(pl.idx ^ nodeGrid[in.k].salt);
     payload[cellStart[compactOf[in.q]] + rankOf[in.q]] = pl;
 }
 
 // Select (rp_subsampling.cuh:70-321, fp_subsampling.cuh:30-306): one output record per occupied
 // cell, written to staging at its dense index (node by node, cells ascending). The record keeps the
 // winner's global quantised position; the colour is the winner's (no filter), the decoded intra
-// sum, the decoded sum of the per-cell sums over the 3x3x3 neighbourhood (inter; equal to the
-// reference's per-point scatter into occupied neighbours because integer sums commute), or the
-// gathered distance-weighted average (kernel_inter_cell_filtering.cuh:109-194) over the points of
-// the neighbourhood, with the weight of a point p for this cell c:
-//   res = float(sqrt(|centre(c) - p|^2) / diag);  w = expf(-(res * res) * 10).
-// The neighbourhood is walked as 9 x-rows: the occupied cells among x-1..x+1 of one row have
-// consecutive dense indices, so each row is one contiguous run of per-cell sums or payloads.
+// sum, or the decoded sum of the per-cell sums over the 3x3x3 neighbourhood (inter; equal to the
+// reference's per-point scatter into occupied neighbours because integer sums commute). The
+// neighbourhood is walked as 9 x-rows: the occupied cells among x-1..x+1 of one row have
+// consecutive dense indices, so each row is one contiguous run of per-cell sums.
 template <typename T, uint32_t MODE>
 __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
                                                 uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
@@ -404,24 +409,20 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
                                                 const uint32_t* __restrict__ cellKey,
                                                 const uint32_t* __restrict__ cellNode,
                                                 const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
-                                                const uint32_t* __restrict__ cellStart,
-                                                const Payload<T>* __restrict__ payload, PointStore<T> ps,
-                                                uint32_t* __restrict__ stageNode)
+                                                PointStore<T> ps, uint32_t* __restrict__ stageNode)
 {
     for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
     {
         const uint32_t c  = static_cast<uint32_t>(ci);
         const uint32_t kl = cellNode[c];
         const NodeGrid ng = nodeGrid[nodeBase + kl];
-        uint64_t best     = MODE == COLOR_INTER_WEIGHTED ? ~0ull : win[c];
         OutRecord rec{};
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
 
         if (MODE == COLOR_INTRA)
         {
             decodeColors(color[c], rec.r, rec.g, rec.b);
         }
-        else if (MODE == COLOR_INTER || MODE == COLOR_INTER_WEIGHTED)
+        else if (MODE == COLOR_INTER)
         {
             const uint32_t key    = cellKey[c];
             const uint32_t gs     = ng.grid.gridSize;
@@ -431,15 +432,11 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
             const int32_t x0 = cx > 0 ? cx - 1 : 0;
             const int32_t x1 = cx + 1 < static_cast<int32_t>(gs) ? cx + 1 : cx;
             const uint32_t span = static_cast<uint32_t>(x1 - x0 + 1);
-
-            // pass 1: first dense index and number of occupied cells of every row
-            uint32_t first[9], count[9];
+            uint64_t sum        = 0;
 #pragma unroll
             for (int r = 0; r < 9; ++r)
             {
                 const int32_t ny = cy + (r / 3) - 1, nz = cz + (r % 3) - 1;
-                count[r] = 0;
-                first[r] = 0;
                 if (ny < 0 || nz < 0 || ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
                     continue;
                 const uint64_t L0 = static_cast<uint64_t>(x0) + static_cast<uint64_t>(ny) * gs +
@@ -449,58 +446,14 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
                 uint64_t v        = bitmap[w0];
                 if (b0 + span > 32)
                     v |= static_cast<uint64_t>(bitmap[w0 + 1]) << 32;
-                const uint32_t m = static_cast<uint32_t>(v >> b0) & ((1u << span) - 1u);
-                count[r]         = __popc(m);
-                first[r]         = prefix[w0] + __popc(static_cast<uint32_t>(v) & ((1u << b0) - 1u));
+                const uint32_t m     = static_cast<uint32_t>(v >> b0) & ((1u << span) - 1u);
+                const uint32_t first = prefix[w0] + __popc(static_cast<uint32_t>(v) & ((1u << b0) - 1u));
+                for (uint32_t i = 0, n = __popc(m); i < n; ++i)
+                    sum += color[first + i];
             }
-            if (MODE == COLOR_INTER)
-            {
-                uint64_t sum = 0;
-#pragma unroll
-                for (int r = 0; r < 9; ++r)
-                    for (uint32_t i = 0; i < count[r]; ++i)
-                        sum += color[first[r] + i];
-                decodeColors(sum, rec.r, rec.g, rec.b);
-            }
-            else
-            {
-                // pass 2: payload runs of the rows
-                uint32_t lo[9], hi[9];
-#pragma unroll
-                for (int r = 0; r < 9; ++r)
-                {
-                    lo[r] = count[r] ? cellStart[first[r]] : 0;
-                    hi[r] = count[r] ? cellStart[first[r] + count[r]] : 0;
-                }
-                const uint32_t cLo = cellStart[c], cHi = cellStart[c + 1];
-                const double t     = ng.grid.bbSize / gs;
-                const double ctrX  = ng.grid.minX + cx * t + (t * 0.5);
-                const double ctrY  = ng.grid.minY + cy * t + (t * 0.5);
-                const double ctrZ  = ng.grid.minZ + cz * t + (t * 0.5);
-                for (int r = 0; r < 9; ++r)
-                {
-                    for (uint32_t q = lo[r]; q < hi[r]; ++q)
-                    {
-                        const Payload<T> pl = payload[q];
-                        if (q >= cLo && q < cHi)
-                        {
-                            const uint64_t k2 = (static_cast<uint64_t>(pl.rank) << 32) | pl.pos;
-                            best              = k2 < best ? k2 : best;
-                        }
-                        const double uX = ctrX - static_cast<double>(pl.x);
-                        const double uY = ctrY - static_cast<double>(pl.y);
-                        const double uZ = ctrZ - static_cast<double>(pl.z);
-                        float res       = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
-                        res             = expf(-((res * res) * 10.f));
-                        acc[0] += static_cast<float>(pl.r) * res;
-                        acc[1] += static_cast<float>(pl.g) * res;
-                        acc[2] += static_cast<float>(pl.b) * res;
-                        acc[3] += res;
-                    }
-                }
-            }
+            decodeColors(sum, rec.r, rec.g, rec.b);
         }
-        const uint32_t wpos = static_cast<uint32_t>(best & 0xFFFFFFFFull);
+        const uint32_t wpos = static_cast<uint32_t>(win[c] & 0xFFFFFFFFull);
         OutRecord wrec      = ps.record(wpos);
         if (MODE == COLOR_NONE)
         {
@@ -511,12 +464,6 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
             rec.x = wrec.x;
             rec.y = wrec.y;
             rec.z = wrec.z;
-            if (MODE == COLOR_INTER_WEIGHTED)
-            {
-                rec.r = static_cast<uint16_t>(acc[0] / acc[3]);
-                rec.g = static_cast<uint16_t>(acc[1] / acc[3]);
-                rec.b = static_cast<uint16_t>(acc[2] / acc[3]);
-            }
         }
         const uint32_t st = cursor - ps.N + c;  // staging index
         storeRecord(ps.stOut + st, rec);
@@ -525,6 +472,112 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
         ps.xyz(wpos, d[0], d[1], d[2]);
         stageNode[st] = ng.id;
     }
+}
+
+// Select with the distance-weighted inter-cell filter (kernel_inter_cell_filtering.cuh:109-194,
+// weighted decode rp_subsampling.cuh:222-230): the gathered average over the points of the 3x3x3
+// neighbourhood, with the weight of a point p for this cell c
+//   res = float(sqrt(|centre(c) - p|^2) / diag);  w = expf(-(res * res) * 10)
+// and colour u16(sum c w / sum w). The neighbourhood is walked as 9 x-rows, each one contiguous
+// run of payloads; the winner is the minimum-rank point of the centre cell's own run, whose
+// coordinates and index are kept on the way. One thread per cell.
+template <typename T, bool RANDOM>
+__global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                        uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
+                                                        const uint32_t* __restrict__ bitmap,
+                                                        const uint32_t* __restrict__ prefix,
+                                                        const uint32_t* __restrict__ cellKey,
+                                                        const uint32_t* __restrict__ cellNode,
+                                                        const uint32_t* __restrict__ cellStart,
+                                                        const Payload<T, RANDOM>* __restrict__ payload,
+                                                        PointStore<T> ps, uint32_t* __restrict__ stageNode)
+{
+    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= cells)
+        return;
+    const uint32_t kl    = cellNode[c];
+    const NodeGrid ng    = nodeGrid[nodeBase + kl];
+    const uint32_t key   = cellKey[c];
+    const uint32_t gs    = ng.grid.gridSize;
+    const uint64_t words = static_cast<uint64_t>(kl) * wordsPerNode;
+    const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>((key / gs) % gs),
+                  cz = static_cast<int32_t>(key / (gs * gs));
+    const int32_t x0 = cx > 0 ? cx - 1 : 0;
+    const int32_t x1 = cx + 1 < static_cast<int32_t>(gs) ? cx + 1 : cx;
+    const uint32_t span = static_cast<uint32_t>(x1 - x0 + 1);
+
+    // first dense index and number of occupied cells of every row
+    uint32_t first[9], count[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+    {
+        const int32_t ny = cy + (r / 3) - 1, nz = cz + (r % 3) - 1;
+        count[r] = 0;
+        first[r] = 0;
+        if (ny < 0 || nz < 0 || ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
+            continue;
+        const uint64_t L0 =
+                static_cast<uint64_t>(x0) + static_cast<uint64_t>(ny) * gs + static_cast<uint64_t>(nz) * gs * gs;
+        const uint64_t w0 = words + (L0 >> 5);
+        const uint32_t b0 = static_cast<uint32_t>(L0 & 31);
+        uint64_t v        = bitmap[w0];
+        if (b0 + span > 32)
+            v |= static_cast<uint64_t>(bitmap[w0 + 1]) << 32;
+        const uint32_t m = static_cast<uint32_t>(v >> b0) & ((1u << span) - 1u);
+        count[r]         = __popc(m);
+        first[r]         = prefix[w0] + __popc(static_cast<uint32_t>(v) & ((1u << b0) - 1u));
+    }
+    uint32_t lo[9], hi[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+    {
+        lo[r] = count[r] ? cellStart[first[r]] : 0;
+        hi[r] = count[r] ? cellStart[first[r] + count[r]] : 0;
+    }
+    const uint32_t cLo = cellStart[c], cHi = cellStart[c + 1];
+    const double t     = ng.grid.bbSize / gs;
+    const double ctrX  = ng.grid.minX + cx * t + (t * 0.5);
+    const double ctrY  = ng.grid.minY + cy * t + (t * 0.5);
+    const double ctrZ  = ng.grid.minZ + cz * t + (t * 0.5);
+    float acc[4]       = {0.f, 0.f, 0.f, 0.f};
+    uint32_t bestRank  = ~0u, bestIdx = 0;
+    T bx = 0, by = 0, bz = 0;
+    for (int r = 0; r < 9; ++r)
+    {
+        for (uint32_t q = lo[r]; q < hi[r]; ++q)
+        {
+            const Payload<T, RANDOM> pl = payload[q];
+            if (q >= cLo && q < cHi && pl.rank() < bestRank)
+            {
+                bestRank = pl.rank();
+                bestIdx  = pl.idx;
+                bx       = pl.x;
+                by       = pl.y;
+                bz       = pl.z;
+            }
+            const double uX = ctrX - static_cast<double>(pl.x);
+            const double uY = ctrY - static_cast<double>(pl.y);
+            const double uZ = ctrZ - static_cast<double>(pl.z);
+            float res       = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
+            res             = expf(-((res * res) * 10.f));
+            acc[0] += static_cast<float>(pl.rgb & 0xFF) * res;
+            acc[1] += static_cast<float>((pl.rgb >> 8) & 0xFF) * res;
+            acc[2] += static_cast<float>((pl.rgb >> 16) & 0xFF) * res;
+            acc[3] += res;
+        }
+    }
+    OutRecord rec = quantizePoint(bx, by, bz, 0, ps.cubic, ps.sx, ps.sy, ps.sz);
+    rec.r         = static_cast<uint16_t>(acc[0] / acc[3]);
+    rec.g         = static_cast<uint16_t>(acc[1] / acc[3]);
+    rec.b         = static_cast<uint16_t>(acc[2] / acc[3]);
+    const uint32_t st = cursor - ps.N + c;  // staging index
+    storeRecord(ps.stOut + st, rec);
+    ps.stLut[st] = bestIdx;
+    T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
+    d[0]         = bx;
+    d[1]         = by;
+    d[2]         = bz;
+    stageNode[st] = ng.id;
 }
 
 // DFS post-order placement (A6, kernel_helpers.cuh:55-59): subtree sums of subsample sizes
