@@ -300,7 +300,7 @@ void OctreeBuilder::countPoints()
     scanExclusive<uint32_t>(nb, ArrayIn{bucketCount}, BucketStartOut{bucketStart, bucketCursor},
                             itsScanScratch.as<uint32_t>(), bucketStart + nb, itsStream);
     OC_LAUNCH_CHECK();
-    const unsigned scatterBlocks = static_cast<unsigned>((static_cast<uint64_t>(n) + SCATTER_CHUNK - 1) / SCATTER_CHUNK);
+    const unsigned scatterBlocks = static_cast<unsigned>((static_cast<uint64_t>(n) + STAGE_CHUNK - 1) / STAGE_CHUNK);
     kBucketScatter<T><<<scatterBlocks, SORT_THREADS, 0, itsStream>>>(itsCloudDevice, n, itsCloud.dataStride, g,
                                                                      3 * itsCoarse, nb, bucketCursor,
                                                                      itsBucketed.as<SortedPoint<T>>(),

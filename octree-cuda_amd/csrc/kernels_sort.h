@@ -6,9 +6,10 @@
 // cell instead (child slot order: x bit 4, y bit 2, z bit 1, as point_count_propagation.cuh:62-70),
 // in two passes that only use LDS atomics per point:
 //   kBucketHist     per-workgroup LDS histogram of coarse buckets (level-lc cells) -> bucket sizes
-//   kBucketScatter  each workgroup ranks its chunk of 16K points per bucket in LDS, reserves one
-//                   range per bucket with a single global atomic and writes 32-byte sorted-point
-//                   records (coordinates, point index, colour) plus the 3*lc-bit fine code
+//   kBucketScatter  each workgroup ranks its chunk of 4096 points per bucket in LDS, reserves one
+//                   range per bucket with a single global atomic, stages the chunk in bucket order
+//                   in LDS and writes 32-byte sorted-point records (coordinates, point index,
+//                   colour) plus the 3*lc-bit fine code as contiguous runs
 //   kFineSort       one workgroup per bucket: LDS histogram over the bucket's (2^lc)^3 level-0
 //                   cells, exclusive scan, dense counts of levels 0..lc of the count pyramid,
 //                   then a scatter into full Morton order
@@ -33,8 +34,8 @@ constexpr uint32_t MAX_FINE_CELLS   = 1u << (3 * MAX_COARSE_LEVEL);    // 32768
 constexpr uint32_t MAX_BUCKETS      = 4096;                            // (512 >> 5)^3
 constexpr int SORT_THREADS          = 1024;
 constexpr int SORT_WAVES            = SORT_THREADS / 64;
-constexpr int SCATTER_ITEMS         = 16;
-constexpr uint32_t SCATTER_CHUNK    = SORT_THREADS * SCATTER_ITEMS;    // points per scatter workgroup
+constexpr int STAGE_ITEMS           = 4;
+constexpr uint32_t STAGE_CHUNK      = SORT_THREADS * STAGE_ITEMS;      // points per scatter workgroup
 constexpr int HIST_THREADS          = 512;
 
 // A point of the sorted cloud: the original coordinates, its index in the input cloud and its
@@ -48,6 +49,7 @@ struct alignas(sizeof(T) == 8 ? 16 : 4) SortedPoint
 };
 static_assert(sizeof(SortedPoint<double>) == 32, "double sorted point is 32 bytes");
 static_assert(sizeof(SortedPoint<float>) == 20, "float sorted point is 20 bytes");
+static_assert(STAGE_CHUNK <= 65535, "staged slots are 16-bit");
 
 // Bit interleaving for coordinates below 2^10 (part1by2 / compact1by2).
 __host__ __device__ __forceinline__ uint32_t spread3(uint32_t v)
@@ -173,6 +175,10 @@ struct BucketStartOut
     }
 };
 
+// Each workgroup takes a chunk of STAGE_CHUNK consecutive points, ranks them per bucket in LDS,
+// reserves one range per occupied bucket with a single global atomic, stages the records in LDS
+// in bucket order and writes them out so that consecutive lanes write consecutive records of one
+// bucket.
 template <typename T>
 __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
                                                                 uint32_t stride, Gridding g, uint32_t fineBits,
@@ -180,55 +186,88 @@ __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __
                                                                 SortedPoint<T>* __restrict__ dst,
                                                                 uint16_t* __restrict__ fineCode)
 {
-    __shared__ uint32_t h[MAX_BUCKETS];
-    for (uint32_t b = threadIdx.x; b < buckets; b += SORT_THREADS)
-        h[b] = 0;
+    __shared__ SortedPoint<T> stage[STAGE_CHUNK];
+    __shared__ uint32_t delta[MAX_BUCKETS];           // counts, then global start - local start
+    __shared__ uint16_t localStart[MAX_BUCKETS + 1];  // first staged slot of every bucket
+    __shared__ uint32_t scratch[SORT_WAVES];
+    for (uint32_t b = threadIdx.x; b < MAX_BUCKETS; b += SORT_THREADS)
+        delta[b] = 0;
     __syncthreads();
 
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * SCATTER_CHUNK;
-    uint32_t code[SCATTER_ITEMS], rank[SCATTER_ITEMS];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * STAGE_CHUNK;
+    uint32_t code[STAGE_ITEMS], rank[STAGE_ITEMS];
+    SortedPoint<T> p[STAGE_ITEMS];
 #pragma unroll
-    for (int j = 0; j < SCATTER_ITEMS; ++j)
+    for (int j = 0; j < STAGE_ITEMS; ++j)
     {
         const uint64_t i = base + static_cast<uint64_t>(j) * SORT_THREADS + threadIdx.x;
         code[j]          = ~0u;
         rank[j]          = 0;
         if (i < n)
         {
-            double x, y, z;
-            loadXYZ<T>(cloud + i * stride, x, y, z);
-            code[j] = pointMorton(x, y, z, g);
+            const uint8_t* rec = cloud + i * stride;
+            __builtin_memcpy(&p[j].x, rec, 3 * sizeof(T));
+            const uint8_t* col = rec + 3 * sizeof(T);
+            p[j].idx           = static_cast<uint32_t>(i);
+            p[j].rgb = static_cast<uint32_t>(col[0]) | (static_cast<uint32_t>(col[1]) << 8) |
+                       (static_cast<uint32_t>(col[2]) << 16);
+            code[j] = pointMorton(static_cast<double>(p[j].x), static_cast<double>(p[j].y),
+                                  static_cast<double>(p[j].z), g);
             if (code[j] != ~0u)
-                rank[j] = atomicAdd(h + (code[j] >> fineBits), 1u);
+                rank[j] = atomicAdd(delta + (code[j] >> fineBits), 1u);
         }
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < buckets; b += SORT_THREADS)
-    {
-        const uint32_t c = h[b];
-        if (c)
-            h[b] = atomicAdd(cursor + b, c);
-    }
-    __syncthreads();
-    const uint32_t fineMask = (1u << fineBits) - 1u;
+
+    // exclusive scan of the bucket counts (MAX_BUCKETS / SORT_THREADS consecutive per thread),
+    // one global reservation per occupied bucket
+    constexpr int PER = MAX_BUCKETS / SORT_THREADS;
+    uint32_t cnt[PER], sum = 0;
 #pragma unroll
-    for (int j = 0; j < SCATTER_ITEMS; ++j)
+    for (int k = 0; k < PER; ++k)
     {
-        if (code[j] == ~0u)
-            continue;
-        const uint64_t i   = base + static_cast<uint64_t>(j) * SORT_THREADS + threadIdx.x;
-        const uint8_t* rec = cloud + i * stride;
-        SortedPoint<T> p;
-        __builtin_memcpy(&p.x, rec, 3 * sizeof(T));
-        const uint8_t* col = rec + 3 * sizeof(T);
-        p.idx              = static_cast<uint32_t>(i);
-        p.rgb              = static_cast<uint32_t>(col[0]) | (static_cast<uint32_t>(col[1]) << 8) |
-                static_cast<uint32_t>(col[2]) << 16;
-        const uint32_t d = h[code[j] >> fineBits] + rank[j];
+        cnt[k] = delta[PER * threadIdx.x + k];
+        sum += cnt[k];
+    }
+    uint32_t total;
+    uint32_t off = sortBlockExclusiveScan(sum, total, scratch);
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+    {
+        const uint32_t b = PER * threadIdx.x + k;
+        localStart[b]    = static_cast<uint16_t>(off);
+        if (cnt[k])
+            delta[b] = atomicAdd(cursor + b, cnt[k]) - off;
+        off += cnt[k];
+    }
+    if (threadIdx.x == 0)
+        localStart[MAX_BUCKETS] = static_cast<uint16_t>(total);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < STAGE_ITEMS; ++j)
+        if (code[j] != ~0u)
+            stage[localStart[code[j] >> fineBits] + rank[j]] = p[j];
+    __syncthreads();
+
+    const uint32_t fineMask = (1u << fineBits) - 1u;
+    for (uint32_t j = threadIdx.x; j < total; j += SORT_THREADS)
+    {
+        uint32_t lo = 0, hi = MAX_BUCKETS;  // the last bucket whose first slot is <= j
+        while (hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (localStart[mid] <= j)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t d = j + delta[lo];
         if (d >= n)
-            continue;  // cannot happen: both passes compute the same codes
-        dst[d]           = p;
-        fineCode[d]      = static_cast<uint16_t>(code[j] & fineMask);
+            continue;  // cannot happen: the histogram and this pass compute the same codes
+        const SortedPoint<T> q = stage[j];
+        dst[d]                 = q;
+        fineCode[d]            = static_cast<uint16_t>(
+                pointMorton(static_cast<double>(q.x), static_cast<double>(q.y), static_cast<double>(q.z), g) & fineMask);
     }
 }
 
