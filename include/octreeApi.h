@@ -133,4 +133,36 @@ EXPORTED int ocpi_copy_output (void* session, uint8_t* destination, uint64_t cou
  * build, [5] summed subsampling insert kernels, [6] summed subsampling emit kernels. */
 EXPORTED int ocpi_get_stage_timings (void* session, float* milliseconds, uint32_t n);
 
+/* ---- build service: continuous batching of many builds on one GPU ---------------------------
+ * None of these has a counterpart in the reference, which builds one cloud per process
+ * (src/main.cpp:55-109). A service owns `slots` workers, each with its own HIP stream and a
+ * device arena that stays resident between requests. A configured session (setters and
+ * configure_* as for ocpi_init_octree) is a request: ocpi_service_submit queues it, a free slot
+ * takes it when the projected device memory of all slots stays within memoryBudget bytes
+ * (footprint estimated from N and the output factor), builds overlap on the device, and the
+ * finished octree is copied to host memory and attached to the session, whose exports and
+ * copies then work as after ocpi_generate_octree. The session and its cloud buffer must stay
+ * untouched until ocpi_service_wait has returned for it. */
+EXPORTED int ocpi_create_service (void** service, int device, uint32_t slots, uint64_t memoryBudget);
+/* Fails requests still queued and waits for the running ones. */
+EXPORTED int ocpi_destroy_service (void* service);
+/* Validates the session's configuration and queues it (-1 if invalid or above the budget). */
+EXPORTED int ocpi_service_submit (void* service, void* session);
+/* Waits up to timeoutMs for the session's build: 0 done, -1 failed or timed out (see last error). */
+EXPORTED int ocpi_service_wait (void* service, void* session, uint32_t timeoutMs);
+/* Number of slots that take requests (1..slots). */
+EXPORTED int ocpi_service_set_active_slots (void* service, uint32_t slots);
+/* stats: [0] submitted [1] completed [2] failed [3] queued now [4] running now [5] max queue depth
+ * [6] p50 latency ms [7] p99 latency ms [8] mean latency ms [9] reserved device bytes
+ * [10] memory budget [11] active slots [12] slots [13] mean queue wait ms [14] max latency ms
+ * [15] peak reserved device bytes. Latency = submit to result attached to the session. */
+EXPORTED int ocpi_service_get_stats (void* service, double* stats, uint32_t n);
+EXPORTED int ocpi_service_reset_stats (void* service);
+/* Of the session's last service request: [0] queue wait ms [1] build + copy ms [2] total ms. */
+EXPORTED int ocpi_get_request_timing (void* session, double* milliseconds, uint32_t n);
+
+/* Device bytes a build of the session's configured cloud may need (the estimate the service admits
+ * requests by; E = N * outputFactor drives most of it). Validates the configuration. */
+EXPORTED int ocpi_estimate_footprint (void* session, uint64_t* bytes);
+
 #endif

@@ -1,0 +1,99 @@
+// The finished octree as the exports and the C ABI read it: either the builder itself (device
+// resident, direct sessions) or a host snapshot of it (builds run by the BuildService, whose
+// builders are reused by the next request).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "types.h"
+
+namespace oc {
+
+struct CloudInfo  // PointCloudInfo (metadata.cuh:52-71)
+{
+    uint32_t pointAmount = 0;
+    uint32_t dataStride  = 0;
+    double bbMin[3]      = {0, 0, 0};
+    double bbMax[3]      = {0, 0, 0};
+    double offset[3]     = {0, 0, 0};
+    double scale[3]      = {1, 1, 1};
+    uint8_t cloudType    = 0;
+};
+
+struct ProcessingInfo  // ProcessingInfo (metadata.cuh:75-88)
+{
+    uint32_t subsamplingGrid  = 128;
+    uint32_t chunkingGrid     = 512;
+    uint32_t mergingThreshold = 10000;
+    bool useIntraCellAvg      = false;
+    bool useInterCellAvg      = false;
+    bool useWeightingFunction = false;
+    bool useReplacementScheme = false;
+    bool useRandomSubsampling = false;
+    float outputFactor        = 2.2f;
+    uint64_t seed             = 0;
+};
+
+// Host copy of the sparse nodes (for export and reports).
+struct HostOctree
+{
+    std::vector<uint32_t> denseKey, count, pointCount;
+    std::vector<uint64_t> dataIdx;
+    std::vector<int32_t> parent, children;
+    std::vector<uint8_t> flags;
+};
+
+class BuildResult
+{
+public:
+    virtual ~BuildResult() = default;
+
+    virtual const CloudInfo& cloud() const           = 0;
+    virtual const ProcessingInfo& processing() const = 0;
+    virtual uint32_t depth() const                   = 0;
+    virtual uint32_t denseNodes() const              = 0;
+    virtual uint32_t sparseNodes() const             = 0;
+    virtual uint64_t capacity() const                = 0;  // E
+    virtual uint64_t used() const                    = 0;  // N + I
+    virtual const std::vector<float>& stageTimings() const = 0;
+
+    virtual void copyNodes(HostOctree& out) const                                     = 0;
+    virtual void copyLut(uint32_t* dst, uint64_t count) const                         = 0;
+    virtual void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const = 0;
+    void copyOutput(uint8_t* dst, uint64_t count) const { copyOutputRange(dst, 0, count); }
+};
+
+// Snapshot of a finished build in host memory: the nodes and the used part of the LUT and the
+// output; records beyond the used part read as zero (the zero tail of octree.bin).
+class HostBuildResult final : public BuildResult
+{
+public:
+    static std::unique_ptr<HostBuildResult> capture(const BuildResult& source);
+
+    const CloudInfo& cloud() const override { return itsCloud; }
+    const ProcessingInfo& processing() const override { return itsProc; }
+    uint32_t depth() const override { return itsDepth; }
+    uint32_t denseNodes() const override { return itsDense; }
+    uint32_t sparseNodes() const override { return itsM; }
+    uint64_t capacity() const override { return itsE; }
+    uint64_t used() const override { return itsUsed; }
+    const std::vector<float>& stageTimings() const override { return itsTimings; }
+
+    void copyNodes(HostOctree& out) const override { out = itsNodes; }
+    void copyLut(uint32_t* dst, uint64_t count) const override;
+    void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
+
+private:
+    CloudInfo itsCloud;
+    ProcessingInfo itsProc;
+    uint32_t itsDepth = 0, itsDense = 0, itsM = 0;
+    uint64_t itsE = 0, itsUsed = 0;
+    std::vector<float> itsTimings;
+    HostOctree itsNodes;
+    std::vector<uint32_t> itsLut;
+    std::vector<uint8_t> itsOut;
+};
+
+}  // namespace oc

@@ -24,43 +24,10 @@
 #include <string>
 #include <vector>
 
+#include "build_result.h"
 #include "types.h"
 
 namespace oc {
-
-struct CloudInfo  // PointCloudInfo (metadata.cuh:52-71)
-{
-    uint32_t pointAmount = 0;
-    uint32_t dataStride  = 0;
-    double bbMin[3]      = {0, 0, 0};
-    double bbMax[3]      = {0, 0, 0};
-    double offset[3]     = {0, 0, 0};
-    double scale[3]      = {1, 1, 1};
-    uint8_t cloudType    = 0;
-};
-
-struct ProcessingInfo  // ProcessingInfo (metadata.cuh:75-88)
-{
-    uint32_t subsamplingGrid  = 128;
-    uint32_t chunkingGrid     = 512;
-    uint32_t mergingThreshold = 10000;
-    bool useIntraCellAvg      = false;
-    bool useInterCellAvg      = false;
-    bool useWeightingFunction = false;
-    bool useReplacementScheme = false;
-    bool useRandomSubsampling = false;
-    float outputFactor        = 2.2f;
-    uint64_t seed             = 0;
-};
-
-// Host copy of the finished octree (for export and reports).
-struct HostOctree
-{
-    std::vector<uint32_t> denseKey, count, pointCount;
-    std::vector<uint64_t> dataIdx;
-    std::vector<int32_t> parent, children;
-    std::vector<uint8_t> flags;
-};
 
 class DeviceBuffer
 {
@@ -89,33 +56,45 @@ private:
     size_t itsBytes = 0;
 };
 
-class OctreeBuilder
+class OctreeBuilder final : public BuildResult
 {
 public:
+    // A builder owns one HIP stream and a grow-only device arena; setup() prepares it for one
+    // cloud (uploading a host cloud or borrowing a device one) and may be called again for the
+    // next cloud, reusing the arena.
+    explicit OctreeBuilder(int device);
     OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
                   const uint8_t* deviceCloud);
-    ~OctreeBuilder();
+    ~OctreeBuilder() override;
     OctreeBuilder(const OctreeBuilder&) = delete;
     OctreeBuilder& operator=(const OctreeBuilder&) = delete;
 
+    void setup(const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
+               const uint8_t* deviceCloud);
     void generate();
 
+    // Device bytes held by the arena, and its release (the next setup grows it again).
+    uint64_t arenaBytes() const;
+    void releaseArena();
+    // Upper bound of the arena a build of this cloud and configuration needs, from the same sizes
+    // setup() and the build allocate (the output capacity E = N * outputFactor drives most terms).
+    static uint64_t estimateFootprint(const CloudInfo& cloud, const ProcessingInfo& proc, bool hostCloud);
+
     bool built() const { return itsBuilt; }
-    const CloudInfo& cloud() const { return itsCloud; }
-    const ProcessingInfo& processing() const { return itsProc; }
-    uint32_t depth() const { return itsDepth; }
-    uint32_t denseNodes() const { return itsDense; }
-    uint32_t sparseNodes() const { return itsM; }
-    uint64_t capacity() const { return itsE; }
-    uint64_t used() const { return itsUsed; }
+    const CloudInfo& cloud() const override { return itsCloud; }
+    const ProcessingInfo& processing() const override { return itsProc; }
+    uint32_t depth() const override { return itsDepth; }
+    uint32_t denseNodes() const override { return itsDense; }
+    uint32_t sparseNodes() const override { return itsM; }
+    uint64_t capacity() const override { return itsE; }
+    uint64_t used() const override { return itsUsed; }
     uint32_t levelStart(uint32_t level) const { return itsLevelStart[level]; }
+    const std::vector<float>& stageTimings() const override { return itsTimings; }
 
     // Copies device results to host (valid after generate()).
-    void copyNodes(HostOctree& out) const;
-    void copyLut(uint32_t* dst, uint64_t count) const;
-    void copyOutput(uint8_t* dst, uint64_t count) const { copyOutputRange(dst, 0, count); }
-    void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const;
-    const std::vector<float>& stageTimings() const { return itsTimings; }
+    void copyNodes(HostOctree& out) const override;
+    void copyLut(uint32_t* dst, uint64_t count) const override;
+    void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
 
 private:
     template <typename T>
@@ -135,6 +114,7 @@ private:
 
     int itsDevice;
     hipStream_t itsStream = nullptr;
+    bool itsReady         = false;  // setup() has run
     CloudInfo itsCloud;
     ProcessingInfo itsProc;
     ColorMode itsColorMode = COLOR_NONE;
@@ -153,7 +133,6 @@ private:
     uint32_t itsM          = 0;
     uint64_t itsE          = 0;
     uint64_t itsUsed       = 0;
-    uint32_t itsInternal   = 0;
     uint32_t itsLevelStart[MAX_LEVELS + 1] = {};
     std::vector<float> itsTimings;
 
@@ -183,6 +162,8 @@ private:
 
     // small device scalars and scan scratch
     DeviceBuffer itsScalars, itsScanScratch;
+
+    std::vector<DeviceBuffer*> arena();
 
     enum EventSlot
     {

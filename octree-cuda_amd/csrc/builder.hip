@@ -103,6 +103,9 @@ size_t DeviceBuffer::peakAllocated()
 // Host orchestration
 // ------------------------------------------------------------------------------------------------
 
+// Bytes of bitmap + prefix the nodes of one subsampling batch may use.
+static constexpr uint64_t kBitmapBudget = 6ull << 30;
+
 static uint32_t octreeDepth(uint32_t grid)  // tools.cu:63-88
 {
     for (uint32_t d = 1; d <= 9; ++d)
@@ -111,11 +114,39 @@ static uint32_t octreeDepth(uint32_t grid)  // tools.cu:63-88
     return 0;
 }
 
-OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc,
-                             const uint8_t* hostCloud, const uint8_t* deviceCloud)
-    : itsDevice(device), itsCloud(cloud), itsProc(proc)
+OctreeBuilder::OctreeBuilder(int device) : itsDevice(device)
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));
+    OC_HIP_CHECK(hipStreamCreateWithFlags(&itsStream, hipStreamNonBlocking));
+    for (auto& ev : itsEvents)
+        OC_HIP_CHECK(hipEventCreate(&ev));
+}
+
+OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc,
+                             const uint8_t* hostCloud, const uint8_t* deviceCloud)
+    : OctreeBuilder(device)
+{
+    setup(cloud, proc, hostCloud, deviceCloud);
+}
+
+static uint64_t coordinateBytes(const CloudInfo& c)
+{
+    return c.cloudType == CLOUD_FLOAT_UINT8_T ? 12 : 24;
+}
+
+static uint64_t sortedPointBytes(const CloudInfo& c)
+{
+    return c.cloudType == CLOUD_FLOAT_UINT8_T ? sizeof(SortedPoint<float>) : sizeof(SortedPoint<double>);
+}
+
+void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
+                          const uint8_t* deviceCloud)
+{
+    OC_HIP_CHECK(hipSetDevice(itsDevice));
+    itsReady = false;
+    itsBuilt = false;
+    itsCloud = cloud;
+    itsProc  = proc;
     itsDepth = octreeDepth(proc.chunkingGrid);
     itsDense = 0;
     for (uint32_t l = 0; l <= itsDepth; ++l)
@@ -134,10 +165,6 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
                                                                 : COLOR_NONE);
     itsE = static_cast<uint64_t>(static_cast<float>(cloud.pointAmount) * proc.outputFactor);  // point_cloud.cuh:24
 
-    OC_HIP_CHECK(hipStreamCreateWithFlags(&itsStream, hipStreamNonBlocking));
-    for (auto& ev : itsEvents)
-        OC_HIP_CHECK(hipEventCreate(&ev));
-
     const uint64_t cloudBytes = static_cast<uint64_t>(cloud.pointAmount) * cloud.dataStride;
     if (deviceCloud)
     {
@@ -146,7 +173,7 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
     else
     {
         itsCloudOwned.ensure(cloudBytes);
-        OC_HIP_CHECK(hipMemcpy(itsCloudOwned.get(), hostCloud, cloudBytes, hipMemcpyHostToDevice));
+        OC_HIP_CHECK(hipMemcpyAsync(itsCloudOwned.get(), hostCloud, cloudBytes, hipMemcpyHostToDevice, itsStream));
         itsCloudDevice = itsCloudOwned.as<uint8_t>();
     }
 
@@ -159,10 +186,9 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
     itsNodeCount.ensure(sizeof(uint32_t) * mBound);
     itsLut.ensure(sizeof(uint32_t) * itsE);
     itsOut.ensure(sizeof(OutRecord) * itsE);
-    const bool isFloat        = cloud.cloudType == CLOUD_FLOAT_UINT8_T;
-    const uint64_t coordBytes = isFloat ? 12 : 24;
-    const uint64_t sortedBytes = isFloat ? sizeof(SortedPoint<float>) : sizeof(SortedPoint<double>);
-    const uint64_t stagedCap  = std::max<uint64_t>(1, itsE - cloud.pointAmount);
+    const uint64_t coordBytes  = coordinateBytes(cloud);
+    const uint64_t sortedBytes = sortedPointBytes(cloud);
+    const uint64_t stagedCap   = std::max<uint64_t>(1, itsE - cloud.pointAmount);
     itsSorted.ensure(sortedBytes * cloud.pointAmount);
     itsBucketed.ensure(sortedBytes * cloud.pointAmount);
     itsFineCode.ensure(sizeof(uint16_t) * cloud.pointAmount);
@@ -176,6 +202,66 @@ OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const Processin
     itsScalars.ensure(sizeof(uint64_t) * SC_COUNT);
     OC_HIP_CHECK(hipMemsetAsync(itsOut.get(), 0, sizeof(OutRecord) * itsE, itsStream));
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+    itsReady = true;
+}
+
+std::vector<DeviceBuffer*> OctreeBuilder::arena()
+{
+    return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsNodeKey,      &itsNodeCount,   &itsNodePoints,
+            &itsNodeData,    &itsNodeParent,  &itsNodeChild,     &itsNodeFlags,    &itsNodeSorted,  &itsNodeAux,
+            &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
+            &itsBucketCount, &itsBucketStart, &itsBucketCursor,  &itsLut,          &itsOut,         &itsStageNode,
+            &itsStLut,       &itsStOut,       &itsStPts,         &itsInternalList, &itsSegCount,    &itsSegSrc,
+            &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
+            &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
+            &itsCellNode,    &itsCellOf,      &itsCompactOf,     &itsRankOf,       &itsPayload,     &itsScalars,
+            &itsScanScratch};
+}
+
+uint64_t OctreeBuilder::arenaBytes() const
+{
+    uint64_t total = 0;
+    for (DeviceBuffer* b : const_cast<OctreeBuilder*>(this)->arena())
+        total += b->capacity();
+    return total;
+}
+
+void OctreeBuilder::releaseArena()
+{
+    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+    for (DeviceBuffer* b : arena())
+        b->release();
+    itsReady = false;
+    itsBuilt = false;
+}
+
+uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const ProcessingInfo& proc, bool hostCloud)
+{
+    const uint32_t depth = octreeDepth(proc.chunkingGrid);
+    uint64_t dense       = 0;
+    for (uint32_t l = 0; l <= depth; ++l)
+    {
+        const uint64_t g = proc.chunkingGrid >> l;
+        dense += g * g * g;
+    }
+    const uint64_t n      = cloud.pointAmount;
+    const uint64_t e      = static_cast<uint64_t>(static_cast<float>(cloud.pointAmount) * proc.outputFactor);
+    const uint64_t staged = e > n ? e - n : 1;
+    const uint64_t mBound = std::min<uint64_t>(dense, (depth + 1) * n);
+    uint64_t bytes        = hostCloud ? n * cloud.dataStride : 0;
+    bytes += 8 * dense;                                     // count pyramid, dense -> sparse
+    bytes += 8 * mBound + 69 * mBound;                      // node keys / counts, node arrays
+    bytes += 22 * e;                                        // lut + output records
+    bytes += (2 * sortedPointBytes(cloud) + 2) * n;         // bucketed, sorted, fine codes
+    bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
+    // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
+    // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
+    uint64_t span = proc.subsamplingGrid;
+    const uint64_t wordsPerNode = (span * span * span + 31) / 32;
+    const uint64_t levelNodes   = std::min<uint64_t>(mBound, n / std::max<uint32_t>(1, proc.mergingThreshold) + 8);
+    bytes += (12 + sizeof(Payload<double, true>)) * e + 32 * staged;
+    bytes += std::min<uint64_t>(kBitmapBudget, 8 * wordsPerNode * levelNodes);
+    return bytes;
 }
 
 OctreeBuilder::~OctreeBuilder()
@@ -223,6 +309,8 @@ V OctreeBuilder::readDevice(const V* devicePtr)
 void OctreeBuilder::generate()
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));
+    if (!itsReady)
+        throw std::runtime_error("the builder has not been set up for a cloud");
     itsBuilt = false;
     if (itsCloud.cloudType == CLOUD_FLOAT_UINT8_T)
         generateTyped<float>();
@@ -457,8 +545,6 @@ static void launchAccumulate(bool random, unsigned blocks, hipStream_t s, uint32
                                                              rankOf);
 }
 
-// Bytes of bitmap + prefix the nodes of one subsampling batch may use.
-static constexpr uint64_t kBitmapBudget = 6ull << 30;
 
 template <typename T>
 void OctreeBuilder::subsampleLevels()
@@ -733,6 +819,7 @@ void OctreeBuilder::placeInternalNodes()
     }
 }
 
+// Copies go through the builder's own stream, so slots of a build service copy concurrently.
 void OctreeBuilder::copyNodes(HostOctree& o) const
 {
     if (!itsBuilt)
@@ -745,27 +832,34 @@ void OctreeBuilder::copyNodes(HostOctree& o) const
     o.parent.resize(m);
     o.children.resize(8 * m);
     o.flags.resize(m);
-    OC_HIP_CHECK(hipMemcpy(o.denseKey.data(), itsNodeKey.get(), 4 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.count.data(), itsNodeCount.get(), 4 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.pointCount.data(), itsNodePoints.get(), 4 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.dataIdx.data(), itsNodeData.get(), 8 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.parent.data(), itsNodeParent.get(), 4 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.children.data(), itsNodeChild.get(), 32 * m, hipMemcpyDeviceToHost));
-    OC_HIP_CHECK(hipMemcpy(o.flags.data(), itsNodeFlags.get(), m, hipMemcpyDeviceToHost));
+    auto copy = [&](void* dst, const void* src, uint64_t bytes) {
+        OC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, itsStream));
+    };
+    copy(o.denseKey.data(), itsNodeKey.get(), 4 * m);
+    copy(o.count.data(), itsNodeCount.get(), 4 * m);
+    copy(o.pointCount.data(), itsNodePoints.get(), 4 * m);
+    copy(o.dataIdx.data(), itsNodeData.get(), 8 * m);
+    copy(o.parent.data(), itsNodeParent.get(), 4 * m);
+    copy(o.children.data(), itsNodeChild.get(), 32 * m);
+    copy(o.flags.data(), itsNodeFlags.get(), m);
+    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 
 void OctreeBuilder::copyLut(uint32_t* dst, uint64_t count) const
 {
     if (count > itsE)
         throw std::runtime_error("copy beyond the output capacity");
-    OC_HIP_CHECK(hipMemcpy(dst, itsLut.get(), 4 * count, hipMemcpyDeviceToHost));
+    OC_HIP_CHECK(hipMemcpyAsync(dst, itsLut.get(), 4 * count, hipMemcpyDeviceToHost, itsStream));
+    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 
 void OctreeBuilder::copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const
 {
     if (first + count > itsE)
         throw std::runtime_error("copy beyond the output capacity");
-    OC_HIP_CHECK(hipMemcpy(dst, itsOut.as<OutRecord>() + first, sizeof(OutRecord) * count, hipMemcpyDeviceToHost));
+    OC_HIP_CHECK(hipMemcpyAsync(dst, itsOut.as<OutRecord>() + first, sizeof(OutRecord) * count,
+                                hipMemcpyDeviceToHost, itsStream));
+    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 
 }  // namespace oc

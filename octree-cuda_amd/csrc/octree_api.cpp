@@ -4,6 +4,8 @@
 #include "octreeApi.h"
 
 #include <algorithm>
+#include <chrono>
+#include <stdexcept>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -11,6 +13,7 @@
 #include "log.h"
 #include "reports.h"
 #include "potree_writer.h"
+#include "service.h"
 #include "session.h"
 
 namespace {
@@ -195,7 +198,7 @@ int ocpi_get_build_summary(void* session, uint64_t* summary, uint32_t n)
 {
     return execute([&] {
         oc::Session* s          = S(session);
-        oc::OctreeBuilder& b    = s->builder();
+        const oc::BuildResult& b = s->result();
         const oc::HostOctree& t = s->hostOctree();
         oc::OctreeInfo info     = oc::nodeStatistics(b, t);
         uint64_t exported       = 0;
@@ -232,19 +235,99 @@ int ocpi_copy_nodes(void* session, uint32_t* denseKey, uint32_t* denseCount, uin
 
 int ocpi_copy_point_lut(void* session, uint32_t* destination, uint64_t count)
 {
-    return execute([&] { S(session)->builder().copyLut(destination, count); });
+    return execute([&] { S(session)->result().copyLut(destination, count); });
 }
 
 int ocpi_copy_output(void* session, uint8_t* destination, uint64_t count)
 {
-    return execute([&] { S(session)->builder().copyOutput(destination, count); });
+    return execute([&] { S(session)->result().copyOutput(destination, count); });
 }
 
 int ocpi_get_stage_timings(void* session, float* milliseconds, uint32_t n)
 {
     return execute([&] {
-        const auto& t = S(session)->builder().stageTimings();
+        const auto& t = S(session)->result().stageTimings();
         for (uint32_t i = 0; i < n; ++i)
             milliseconds[i] = i < t.size() ? t[i] : 0.f;
+    });
+}
+
+// ---- build service (continuous batching) ------------------------------------------------------
+
+namespace {
+
+oc::BuildService* V(void* h)
+{
+    return oc::BuildService::fromHandle(h);
+}
+
+}  // namespace
+
+int ocpi_create_service(void** service, int device, uint32_t slots, uint64_t memoryBudget)
+{
+    return execute([&] {
+        if (!service)
+            throw std::runtime_error("service handle pointer is null");
+        *service = new oc::BuildService(device, slots, memoryBudget);
+    });
+}
+
+int ocpi_destroy_service(void* service)
+{
+    return execute([&] { delete V(service); });
+}
+
+int ocpi_service_submit(void* service, void* session)
+{
+    return execute([&] { V(service)->submit(S(session)); });
+}
+
+int ocpi_service_wait(void* service, void* session, uint32_t timeoutMs)
+{
+    return execute([&] {
+        if (!V(service)->wait(S(session), timeoutMs))
+            throw std::runtime_error("timed out waiting for the build");
+    });
+}
+
+int ocpi_service_set_active_slots(void* service, uint32_t slots)
+{
+    return execute([&] { V(service)->setActiveSlots(slots); });
+}
+
+int ocpi_service_get_stats(void* service, double* stats, uint32_t n)
+{
+    return execute([&] {
+        const std::vector<double> v = V(service)->stats();
+        for (uint32_t i = 0; i < n; ++i)
+            stats[i] = i < v.size() ? v[i] : 0.0;
+    });
+}
+
+int ocpi_service_reset_stats(void* service)
+{
+    return execute([&] { V(service)->resetStats(); });
+}
+
+int ocpi_get_request_timing(void* session, double* milliseconds, uint32_t n)
+{
+    return execute([&] {
+        auto& r = S(session)->request();
+        if (r.state != oc::Session::Request::DONE && r.state != oc::Session::Request::FAILED)
+            throw std::runtime_error("the session has no finished service request");
+        using ms     = std::chrono::duration<double, std::milli>;
+        double v[3] = {ms(r.started - r.submitted).count(), ms(r.finished - r.started).count(),
+                       ms(r.finished - r.submitted).count()};
+        for (uint32_t i = 0; i < n; ++i)
+            milliseconds[i] = i < 3 ? v[i] : 0.0;
+    });
+}
+
+int ocpi_estimate_footprint(void* session, uint64_t* bytes)
+{
+    return execute([&] {
+        oc::Session* s = S(session);
+        s->validate();
+        *bytes = oc::OctreeBuilder::estimateFootprint(s->cloudInfo(), s->processingInfo(), s->hostCloud() != nullptr);
     });
 }

@@ -48,7 +48,7 @@ std::vector<HierarchyEntry> hierarchyEntries(const HostOctree& tree, uint64_t* e
     return entries;
 }
 
-static void writeMetadata(const std::string& path, const OctreeBuilder& b, uint32_t exportedPoints,
+static void writeMetadata(const std::string& path, const BuildResult& b, uint32_t exportedPoints,
                           uint32_t exportedNodes)
 {
     const CloudInfo& c      = b.cloud();
@@ -137,7 +137,7 @@ static void writeMetadata(const std::string& path, const OctreeBuilder& b, uint3
     f << md.dump(4);
 }
 
-void writePotree(const std::string& directory, const OctreeBuilder& b, const HostOctree& tree)
+void writePotree(const std::string& directory, const BuildResult& b, const HostOctree& tree)
 {
     uint64_t exportedPoints = 0;
     std::vector<HierarchyEntry> entries = hierarchyEntries(tree, &exportedPoints);

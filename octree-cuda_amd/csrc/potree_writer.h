@@ -5,7 +5,7 @@
 #include <string>
 #include <vector>
 
-#include "builder.h"
+#include "build_result.h"
 
 namespace oc {
 
@@ -13,6 +13,6 @@ namespace oc {
 // the 22-byte entries; *exportedPoints gets the sum of their point counts.
 std::vector<HierarchyEntry> hierarchyEntries(const HostOctree& tree, uint64_t* exportedPoints);
 
-void writePotree(const std::string& directory, const OctreeBuilder& builder, const HostOctree& tree);
+void writePotree(const std::string& directory, const BuildResult& builder, const HostOctree& tree);
 
 }  // namespace oc

@@ -1,5 +1,7 @@
 #include "reports.h"
 
+#include "builder.h"
+
 #include <algorithm>
 #include <cmath>
 #include <fstream>
@@ -12,7 +14,7 @@ namespace oc {
 
 // updateNodeStatistics / evaluateNodeProperties / calculatePointVarianceInLeafNoes
 // (octree.cu:79-157): recursion from the root over the children of internal nodes.
-OctreeInfo nodeStatistics(const OctreeBuilder& b, const HostOctree& tree)
+OctreeInfo nodeStatistics(const BuildResult& b, const HostOctree& tree)
 {
     OctreeInfo s;
     s.depth            = b.depth();
@@ -57,7 +59,7 @@ OctreeInfo nodeStatistics(const OctreeBuilder& b, const HostOctree& tree)
     return s;
 }
 
-void writeJsonReport(const std::string& filename, const OctreeBuilder& b, const HostOctree& tree)
+void writeJsonReport(const std::string& filename, const BuildResult& b, const HostOctree& tree)
 {
     const ProcessingInfo& p = b.processing();
     OctreeInfo s            = nodeStatistics(b, tree);
@@ -110,7 +112,7 @@ void writeJsonReport(const std::string& filename, const OctreeBuilder& b, const 
 // histogram.cu:39-119. A bin width of 0 selects 3.5 * stdev / leaves^(1/3) (Scott's rule); the
 // reference sizes the bins as (max - min) / width and indexes the maximum one past the end, so
 // one extra bin is kept here.
-void writeHistogram(const std::string& filename, uint32_t binWidth, const OctreeBuilder& b, const HostOctree& tree)
+void writeHistogram(const std::string& filename, uint32_t binWidth, const BuildResult& b, const HostOctree& tree)
 {
     OctreeInfo s = nodeStatistics(b, tree);
     if (binWidth == 0)

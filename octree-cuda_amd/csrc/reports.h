@@ -5,7 +5,7 @@
 #include <cstdint>
 #include <string>
 
-#include "builder.h"
+#include "build_result.h"
 
 namespace oc {
 
@@ -23,10 +23,10 @@ struct OctreeInfo  // metadata.cuh (OctreeInfo)
     uint32_t nodeAmountDense    = 0;
 };
 
-OctreeInfo nodeStatistics(const OctreeBuilder& b, const HostOctree& tree);
+OctreeInfo nodeStatistics(const BuildResult& b, const HostOctree& tree);
 
-void writeJsonReport(const std::string& filename, const OctreeBuilder& b, const HostOctree& tree);
-void writeHistogram(const std::string& filename, uint32_t binWidth, const OctreeBuilder& b, const HostOctree& tree);
+void writeJsonReport(const std::string& filename, const BuildResult& b, const HostOctree& tree);
+void writeHistogram(const std::string& filename, uint32_t binWidth, const BuildResult& b, const HostOctree& tree);
 void writeMemoryReport(const std::string& filename);
 
 }  // namespace oc

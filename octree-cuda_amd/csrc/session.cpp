@@ -10,6 +10,7 @@
 #include "log.h"
 #include "potree_writer.h"
 #include "reports.h"
+#include "service.h"
 
 namespace oc {
 
@@ -39,6 +40,9 @@ Session::Session(int device) : itsDevice(device)
 
 Session::~Session()
 {
+    if (itsRequest.owner &&
+        (itsRequest.state == Request::QUEUED || itsRequest.state == Request::RUNNING))
+        itsRequest.owner->forget(this);
     if (!itsMemoryReport.empty())
     {
         try
@@ -165,35 +169,39 @@ void Session::validate() const
 void Session::initOctree()
 {
     validate();
-    itsBuilder.reset();
-    itsBuilder = std::make_unique<OctreeBuilder>(itsDevice, itsCloud, itsProc, itsHostCloud, itsDeviceCloud);
+    if (!itsBuilder)
+        itsBuilder = std::make_unique<OctreeBuilder>(itsDevice);
+    itsResult = nullptr;
+    itsBuilder->setup(itsCloud, itsProc, itsHostCloud, itsDeviceCloud);
 }
 
 void Session::generateOctree()
 {
     if (!itsBuilder)
         throw std::runtime_error("ocpi_init_octree has not been called");
+    itsResult = nullptr;
     itsBuildSerial++;
     itsBuilder->generate();
+    itsResult = itsBuilder.get();
 }
 
-OctreeBuilder& Session::builder()
+void Session::adoptResult(std::unique_ptr<HostBuildResult> result)
 {
-    if (!itsBuilder || !itsBuilder->built())
-        throw std::runtime_error("the octree has not been generated");
-    return *itsBuilder;
+    itsSnapshot = std::move(result);
+    itsBuildSerial++;
+    itsResult = itsSnapshot.get();
 }
 
-const OctreeBuilder& Session::builder() const
+const BuildResult& Session::result() const
 {
-    if (!itsBuilder || !itsBuilder->built())
+    if (!itsResult)
         throw std::runtime_error("the octree has not been generated");
-    return *itsBuilder;
+    return *itsResult;
 }
 
 const HostOctree& Session::hostOctree()
 {
-    OctreeBuilder& b = builder();
+    const BuildResult& b = result();
     if (!itsHost || itsHostSerial != itsBuildSerial)
     {
         itsHost = std::make_unique<HostOctree>();
@@ -205,12 +213,12 @@ const HostOctree& Session::hostOctree()
 
 void Session::exportPotree(const std::string& directory)
 {
-    writePotree(directory, builder(), hostOctree());
+    writePotree(directory, result(), hostOctree());
 }
 
 void Session::exportJsonReport(const std::string& filename)
 {
-    writeJsonReport(filename, builder(), hostOctree());
+    writeJsonReport(filename, result(), hostOctree());
 }
 
 void Session::exportMemoryReport(const std::string& filename)
@@ -220,7 +228,7 @@ void Session::exportMemoryReport(const std::string& filename)
 
 void Session::exportDistributionHistogram(const std::string& filename, uint32_t binWidth)
 {
-    writeHistogram(filename, binWidth, builder(), hostOctree());
+    writeHistogram(filename, binWidth, result(), hostOctree());
 }
 
 }  // namespace oc

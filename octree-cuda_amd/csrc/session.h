@@ -3,6 +3,7 @@
 // validates them at init, owns the OctreeBuilder, and runs the exports.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -42,12 +43,38 @@ public:
     void exportMemoryReport(const std::string& filename);
     void exportDistributionHistogram(const std::string& filename, uint32_t binWidth);
 
-    OctreeBuilder& builder();
-    const OctreeBuilder& builder() const;
+    // The last finished build: the session's own builder, or the snapshot of a build the
+    // BuildService ran for it.
+    const BuildResult& result() const;
     const HostOctree& hostOctree();  // lazily copied after each build
 
-private:
+    // BuildService hooks. A submitted session is owned by the service until its wait returns.
     void validate() const;
+    const CloudInfo& cloudInfo() const { return itsCloud; }
+    const ProcessingInfo& processingInfo() const { return itsProc; }
+    const uint8_t* hostCloud() const { return itsHostCloud; }
+    const uint8_t* deviceCloud() const { return itsDeviceCloud; }
+    void adoptResult(std::unique_ptr<HostBuildResult> result);
+
+    enum class Request
+    {
+        NONE,
+        QUEUED,
+        RUNNING,
+        DONE,
+        FAILED
+    };
+    struct RequestInfo
+    {
+        Request state = Request::NONE;
+        std::string error;
+        uint64_t footprint = 0;
+        class BuildService* owner = nullptr;  // the service the session was submitted to
+        std::chrono::steady_clock::time_point submitted, started, finished;
+    };
+    RequestInfo& request() { return itsRequest; }
+
+private:
 
     int itsDevice;
     const uint8_t* itsHostCloud   = nullptr;
@@ -55,7 +82,10 @@ private:
     CloudInfo itsCloud;
     ProcessingInfo itsProc;
     std::unique_ptr<OctreeBuilder> itsBuilder;
+    std::unique_ptr<HostBuildResult> itsSnapshot;
+    const BuildResult* itsResult = nullptr;
     std::unique_ptr<HostOctree> itsHost;
+    RequestInfo itsRequest;
     uint64_t itsBuildSerial     = 0;
     uint64_t itsHostSerial      = ~0ull;
     std::string itsMemoryReport;

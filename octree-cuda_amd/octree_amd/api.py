@@ -48,15 +48,31 @@ SIGNATURES = {
     "ocpi_copy_point_lut": [_vp, _vp, _u64],
     "ocpi_copy_output": [_vp, _vp, _u64],
     "ocpi_get_stage_timings": [_vp, _vp, _u32],
+    "ocpi_create_service": [ctypes.POINTER(_vp), ctypes.c_int, _u32, _u64],
+    "ocpi_destroy_service": [_vp],
+    "ocpi_service_submit": [_vp, _vp],
+    "ocpi_service_wait": [_vp, _vp, _u32],
+    "ocpi_service_set_active_slots": [_vp, _u32],
+    "ocpi_service_get_stats": [_vp, _vp, _u32],
+    "ocpi_service_reset_stats": [_vp],
+    "ocpi_get_request_timing": [_vp, _vp, _u32],
+    "ocpi_estimate_footprint": [_vp, _vp],
 }
 
-REFERENCE_ENTRY_POINTS = [n for n in SIGNATURES if n not in (
+SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
+                        "ocpi_service_set_active_slots", "ocpi_service_get_stats", "ocpi_service_reset_stats",
+                        "ocpi_get_request_timing", "ocpi_estimate_footprint"]
+
+REFERENCE_ENTRY_POINTS = [n for n in SIGNATURES if n not in [
     "ocpi_set_point_cloud_device", "ocpi_set_random_seed", "ocpi_get_last_error", "ocpi_get_build_summary",
-    "ocpi_copy_nodes", "ocpi_copy_point_lut", "ocpi_copy_output", "ocpi_get_stage_timings")]
+    "ocpi_copy_nodes", "ocpi_copy_point_lut", "ocpi_copy_output", "ocpi_get_stage_timings"] + SERVICE_ENTRY_POINTS]
 
 SUMMARY_FIELDS = ("sparse_nodes", "points", "capacity", "used", "internal_nodes", "leaf_nodes", "depth",
                   "dense_nodes", "exported_nodes")
 TIMING_FIELDS = ("count_ms", "merge_ms", "distribute_ms", "subsample_ms", "build_ms", "insert_ms", "emit_ms")
+SERVICE_STAT_FIELDS = ("submitted", "completed", "failed", "queued", "running", "max_queue_depth", "p50_ms", "p99_ms",
+                       "mean_ms", "reserved_bytes", "budget_bytes", "active_slots", "slots", "mean_wait_ms",
+                       "max_ms", "peak_reserved_bytes")
 
 _api = None
 
@@ -211,6 +227,60 @@ class Session:
         out = np.empty(count, OUT_DTYPE)
         _check(self._lib.ocpi_copy_output(self.handle, out.ctypes.data, count), "ocpi_copy_output")
         return out
+
+    def footprint(self) -> int:
+        v = ctypes.c_uint64()
+        _check(self._lib.ocpi_estimate_footprint(self.handle, ctypes.byref(v)), "ocpi_estimate_footprint")
+        return int(v.value)
+
+    def request_timing(self) -> dict:
+        v = np.zeros(3, np.float64)
+        _check(self._lib.ocpi_get_request_timing(self.handle, v.ctypes.data, 3), "ocpi_get_request_timing")
+        return {"queue_ms": float(v[0]), "run_ms": float(v[1]), "total_ms": float(v[2])}
+
+
+class Service:
+    """Continuous batching of builds (ocpi_create_service): configured sessions are requests."""
+
+    def __init__(self, device: int = 0, slots: int = 4, memory_budget: int = 200 << 30):
+        self._lib = library()
+        h = _vp()
+        _check(self._lib.ocpi_create_service(ctypes.byref(h), device, slots, memory_budget), "ocpi_create_service")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _check(self._lib.ocpi_destroy_service(self.handle), "ocpi_destroy_service")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, session: Session):
+        _check(self._lib.ocpi_service_submit(self.handle, session.handle), "ocpi_service_submit")
+
+    def wait(self, session: Session, timeout_ms: int = 600_000):
+        _check(self._lib.ocpi_service_wait(self.handle, session.handle, timeout_ms), "ocpi_service_wait")
+
+    def set_active_slots(self, slots: int):
+        _check(self._lib.ocpi_service_set_active_slots(self.handle, slots), "ocpi_service_set_active_slots")
+
+    def stats(self) -> dict:
+        v = np.zeros(len(SERVICE_STAT_FIELDS), np.float64)
+        _check(self._lib.ocpi_service_get_stats(self.handle, v.ctypes.data, len(v)), "ocpi_service_get_stats")
+        return {k: float(x) for k, x in zip(SERVICE_STAT_FIELDS, v)}
+
+    def reset_stats(self):
+        _check(self._lib.ocpi_service_reset_stats(self.handle), "ocpi_service_reset_stats")
 
 
 def set_logging_level(level: int):

@@ -49,6 +49,7 @@ def test_library_exports_exactly_the_declared_symbols():
 def test_binding_covers_every_symbol():
     assert set(api.SIGNATURES) == declared()
     assert set(api.REFERENCE_ENTRY_POINTS) == REFERENCE_API
+    assert set(api.SERVICE_ENTRY_POINTS) <= declared()
     lib = api.library()
     for name in declared():
         assert hasattr(lib, name)
@@ -70,6 +71,24 @@ def test_null_session_returns_error(name):
             args.append(1)
     assert getattr(lib, name)(*args) == -1
     assert "Session" in api.last_error()
+
+
+@pytest.mark.parametrize("name", ["ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
+                                  "ocpi_service_set_active_slots", "ocpi_service_get_stats",
+                                  "ocpi_service_reset_stats"])
+def test_null_service_returns_error(name):
+    lib = api.library()
+    args = [None] + [None if t is ctypes.c_void_p else 1 for t in api.SIGNATURES[name][1:]]
+    assert getattr(lib, name)(*args) == -1
+    assert "BuildService" in api.last_error() or "Session" in api.last_error()
+
+
+def test_create_service_reports_missing_device():
+    lib = api.library()
+    h = ctypes.c_void_p()
+    assert lib.ocpi_create_service(ctypes.byref(h), 12345, 2, 1 << 30) == -1
+    assert "device" in api.last_error()
+    assert lib.ocpi_create_service(ctypes.byref(h), 0, 0, 1 << 30) == -1
 
 
 def test_create_session_reports_missing_device():

@@ -1,0 +1,131 @@
+"""The build service (continuous batching) on the GPU: every request's result equals the same
+request built alone, the memory budget is respected, and failures stay with their request.
+
+Equality: node keys, counts, flags, links, point counts, data offsets, used size and per-node
+point sets and positions exactly; colours exactly, except the distance-weighted filter, whose
+float sums depend on the order points arrive in (+-1 per channel, as against the oracle).
+"""
+import numpy as np
+import pytest
+
+from octree_amd import api
+from octree_amd import workloads as W
+from parity import fetch, node_of_position, records_by_node, synthetic
+
+pytestmark = pytest.mark.gpu
+
+MODES = [(False, False, False, False), (True, False, False, True), (False, True, False, False),
+         (False, True, True, False), (False, True, True, True), (True, False, False, False)]
+
+
+def _requests():
+    out = []
+    sizes = [20_000, 150_000, 45_000, 400_000, 80_000, 260_000]
+    for i, (n, mode) in enumerate(zip(sizes, MODES)):
+        cloud_type = i % 2
+        cloud, spec, cubic = synthetic(n, cloud_type, box=(10.0 + 7 * i, 12.0, 5.0), clusters=3, seed=30 + i,
+                                       offset=0.0 if cloud_type == 0 else 2e5)
+        intra, inter, weighted, rnd = mode
+        build = W.BuildSpec(64 if i % 3 else 128, 2000 + 500 * i, 3.0, 32 if i % 2 else 64, intra=intra,
+                            inter=inter, weighted=weighted, random=rnd, seed=100 + i)
+        out.append((cloud, spec, cubic, build))
+    return out
+
+
+def _session(req):
+    cloud, spec, cubic, build = req
+    s = api.Session(0)
+    s.set_cloud(cloud, spec.points, spec.stride, spec.cloud_type, cubic, [spec.scale] * 3)
+    s.configure(build)
+    return s
+
+
+def _same(a, b, weighted):
+    summ_a, nodes_a, lut_a, out_a = a
+    summ_b, nodes_b, lut_b, out_b = b
+    assert summ_a == summ_b
+    for f in ("key", "count", "flags", "parent", "children", "point_count", "data_idx"):
+        np.testing.assert_array_equal(getattr(nodes_a, f), getattr(nodes_b, f), f)
+    fin = nodes_a.finished
+    used = summ_a["used"]
+    owner = node_of_position(nodes_a.data_idx, nodes_a.point_count, fin, used)
+    ga, ia, ca = records_by_node(owner, lut_a, out_a)
+    gb, ib, cb = records_by_node(owner, lut_b, out_b)
+    np.testing.assert_array_equal(ga, gb)
+    np.testing.assert_array_equal(ia, ib, "per-node point sets")
+    np.testing.assert_array_equal(ca[:, :3], cb[:, :3], "positions")
+    d = np.abs(ca[:, 3:] - cb[:, 3:]).max(initial=0)
+    assert d <= (1 if weighted else 0), f"colours differ by {d}"
+
+
+def test_service_results_equal_builds_alone():
+    reqs = _requests()
+    alone = []
+    for r in reqs:
+        s = _session(r)
+        s.init()
+        s.generate()
+        alone.append(fetch(s))
+        s.close()
+    print("case service mix of", len(reqs), "requests, 3 slots", flush=True)
+    with api.Service(0, slots=3, memory_budget=64 << 30) as svc:
+        sessions = [_session(r) for r in reqs]
+        for s in sessions:
+            svc.submit(s)
+        for s in sessions:
+            svc.wait(s)
+        st = svc.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["p99_ms"] >= st["p50_ms"] > 0
+        for r, s, a in zip(reqs, sessions, alone):
+            _same(fetch(s), a, weighted=r[3].weighted)
+            t = s.request_timing()
+            assert t["total_ms"] >= t["run_ms"] > 0
+            s.close()
+
+
+def test_service_keeps_the_memory_budget():
+    reqs = _requests()[:4]
+    sessions = [_session(r) for r in reqs]
+    # room for about one request at a time
+    need = max(s.footprint() for s in sessions)
+    budget = int(need * 1.3)
+    print("case service budget", budget, flush=True)
+    with api.Service(0, slots=3, memory_budget=budget) as svc:
+        for s in sessions:
+            svc.submit(s)
+        for s in sessions:
+            svc.wait(s)
+        st = svc.stats()
+        assert st["completed"] == len(reqs)
+        assert 0 < st["peak_reserved_bytes"] <= budget
+    for s in sessions:
+        s.close()
+
+
+def test_service_rejects_requests_above_the_budget():
+    req = _requests()[0]
+    s = _session(req)
+    with api.Service(0, slots=2, memory_budget=1 << 20) as svc:
+        with pytest.raises(api.OctreeError, match="budget"):
+            svc.submit(s)
+    s.close()
+
+
+def test_service_failure_stays_with_its_request():
+    reqs = _requests()[:3]
+    cloud, spec, cubic, build = reqs[1]
+    import dataclasses
+    bad = (cloud, spec, cubic, dataclasses.replace(build, output_factor=1.0))  # N + I > E
+    sessions = [_session(reqs[0]), _session(bad), _session(reqs[2])]
+    with api.Service(0, slots=2, memory_budget=64 << 30) as svc:
+        for s in sessions:
+            svc.submit(s)
+        svc.wait(sessions[0])
+        with pytest.raises(api.OctreeError, match="output"):
+            svc.wait(sessions[1])
+        svc.wait(sessions[2])
+        st = svc.stats()
+        assert st["completed"] == 2 and st["failed"] == 1
+    for s in sessions:
+        s.close()
