@@ -10,7 +10,9 @@ inter-cell colour filtering, output factor 2.2.
 
 Prints one JSON line. `roofline` is for the dominant kernel class, timed with HIP events on the
 session stream by the library; `cpu_baseline` times the CPU oracle on a bounded prefix of the same
-cloud on this host.
+cloud on this host. `batched` is the batched-build workload (256 clouds of 10K-2M points through
+the build service, slot count chosen by the tuner, against the same clouds through one slot) with
+its p50/p99 latency; `serving` is a short sweep of Poisson arrival rates through the service.
 """
 import argparse
 import json
@@ -70,17 +72,65 @@ def cpu_baseline(cloud, wl, cubic, sample_points, threads):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_binding as O
 
+    import dataclasses
     s = wl.cloud
     sample = cloud[: sample_points * s.stride]
+    # a prefix subsamples relatively more than the whole cloud, so give it room (the output
+    # capacity only sizes buffers; the work is the same)
+    spec = dataclasses.replace(wl.build, output_factor=max(wl.build.output_factor, 3.0))
     t0 = time.perf_counter()
-    r = O.build(sample, sample_points, s.stride, s.cloud_type, cubic, [s.scale] * 3, wl.build, threads=threads,
+    r = O.build(sample, sample_points, s.stride, s.cloud_type, cubic, [s.scale] * 3, spec, threads=threads,
                 want_data=False)
     dt = time.perf_counter() - t0
     if r.status != 0:
+        log(f"bench: CPU oracle failed with status {r.status}")
         return None
     return {"value": sample_points / dt, "unit": "points/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (std::thread x {threads}) building the first {sample_points:,} points of the "
                       f"same cloud with the same bounding box and configuration once, {dt:.1f} s"}
+
+
+def run_batched(args):
+    from octree_amd import batch
+    log("bench: batched builds: generating 256 clouds")
+    reqs = batch.batch_requests()
+    svc = api.Service(args.device, slots=8, memory_budget=200 << 30)
+    try:
+        best, tuned = batch.tune_slots(svc, reqs)
+        log(f"bench: slot tuner {dict((k, round(v / 1e6, 1)) for k, v in tuned.items())} Mpts/s -> {best} slots")
+        seq = batch.run_batch(svc, reqs, 1)
+        par = batch.run_batch(svc, reqs, best)
+        log(f"bench: sequential {seq}")
+        log(f"bench: batched {par}")
+    finally:
+        svc.close()
+    return {"workload": "256 clouds, 10K-2M points log-uniform (seed 4), f32/f64 mixed, grids 256/64, T 5000, "
+                        "implicit + inter, output factor 3.0",
+            "points_per_s": par["points_per_s"], "clouds_per_s": par["clouds_per_s"], "p50_ms": par["p50_ms"],
+            "p99_ms": par["p99_ms"], "slots": best, "tuner_points_per_s": {str(k): v for k, v in tuned.items()},
+            "sequential_points_per_s": seq["points_per_s"], "sequential_p50_ms": seq["p50_ms"],
+            "sequential_p99_ms": seq["p99_ms"], "speedup_vs_sequential": par["points_per_s"] / seq["points_per_s"],
+            "failed": par["failed"] + seq["failed"]}
+
+
+def run_serving(args):
+    from octree_amd import batch
+    rates = [float(r) for r in args.serving_rates.split(",") if r]
+    log("bench: serving: generating the request clouds")
+    pool = batch.serving_pool()
+    svc = api.Service(args.device, slots=4, memory_budget=200 << 30)
+    out = []
+    try:
+        for rate in rates:
+            reqs = batch.serving_requests(pool, rate, args.serving_seconds)
+            r = batch.run_serving(svc, reqs, args.serving_seconds)
+            log(f"bench: serving {rate} req/s: {r}")
+            out.append(r)
+    finally:
+        svc.close()
+    return {"workload": "Poisson arrivals, 50K-20M points log-uniform (seed 5), f32/f64 50/50, method and filter "
+                        "uniform, grids 512/128, 200 GB budget, 4 slots",
+            "seconds_per_rate": args.serving_seconds, "rates": out}
 
 
 def main():
@@ -89,8 +139,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="C3", choices=sorted(W.WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=-1,
+    ap.add_argument("--cpu-sample", type=int, default=16_000_000,
                     help="points of the cloud the CPU oracle baseline builds (-1 = all, 0 = skip)")
+    ap.add_argument("--batched", type=int, default=1, help="also run the batched-build workload (0 = skip)")
+    ap.add_argument("--serving-rates", default="10,20,40,80",
+                    help="Poisson arrival rates (req/s) of the serving sweep, '' to skip")
+    ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, help="also run a plain D2D copy of this many bytes")
     args = ap.parse_args()
@@ -155,6 +209,10 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
 
+    sess.close()
+    batched = run_batched(args) if args.batched else None
+    serving = run_serving(args) if args.serving_rates else None
+
     cpu = None
     if args.cpu_sample != 0:
         threads = min(os.cpu_count() or 1, 16)
@@ -183,9 +241,10 @@ def main():
                    "output_records": summ["used"], "stage_ms": {k: round(v, 4) for k, v in avg.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    sess.close()
 
 
 if __name__ == "__main__":
