@@ -52,6 +52,19 @@ def algorithmic_bytes(kind, n, stride, grid, summ):
     raise ValueError(kind)
 
 
+def profiled_traffic(workload):
+    """HBM bytes per build and stage from the committed FETCH_SIZE / WRITE_SIZE summary of this
+    workload (calibrated against a plain copy), or None when there is none."""
+    path = os.path.join(REPO, "profiles", f"{workload.lower()}_traffic_fetch_write.md")
+    if not os.path.exists(path):
+        return None
+    for text in open(path):
+        if text.startswith("Machine-readable (bytes per build"):
+            pairs = text.split(":", 1)[1].split(",")
+            return {k.strip(): int(v) for k, v in (p.split("=") for p in pairs)}
+    return None
+
+
 def copy_calibration(nbytes):
     """A plain device-to-device copy of known size (for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
     import ctypes
@@ -198,8 +211,12 @@ def main():
 
     dominant = max(stage_ms, key=stage_ms.get)
     rl_bytes, achieved = rl(dominant)
+    traffic = profiled_traffic(args.workload)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get(dominant) if traffic else None,
+                "traffic_stages": traffic,
+                "traffic_source": f"profiles/{args.workload.lower()}_traffic_fetch_write.md" if traffic else None,
                 "bytes_per_launch": rl_bytes, "avg_launch_ms": round(stage_ms[dominant], 4),
                 "stages": {k: {"ms": round(v, 4), "bytes": rl(k)[0], "GBs": round(rl(k)[1], 1),
                                "frac": round(rl(k)[1] / HBM_PEAK_GBS, 4)} for k, v in stage_ms.items()},
