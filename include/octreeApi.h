@@ -165,4 +165,17 @@ EXPORTED int ocpi_get_request_timing (void* session, double* milliseconds, uint3
  * requests by; E = N * outputFactor drives most of it). Validates the configuration. */
 EXPORTED int ocpi_estimate_footprint (void* session, uint64_t* bytes);
 
+/* Bounding box of a cloud already in HBM, reduced on the GPU: realBB = {min x, y, z, max x, y, z}
+ * of the coordinates (widened to double), cubicBB = the cube the reference CLI builds from it
+ * (src/boundingbox.h:10-52, replacing its host loop). No counterpart in the library API; it
+ * serves callers such as the CLI that would otherwise scan the cloud on the host. */
+EXPORTED int ocpi_compute_bounding_box (
+        const uint8_t* devicePointCloud,
+        uint32_t pointAmount,
+        uint32_t dataStride,
+        uint8_t cloudType,
+        int device,
+        double* realBB,
+        double* cubicBB);
+
 #endif

@@ -10,6 +10,7 @@
 #include <exception>
 #include <string>
 
+#include "bounds.h"
 #include "log.h"
 #include "reports.h"
 #include "potree_writer.h"
@@ -329,5 +330,16 @@ int ocpi_estimate_footprint(void* session, uint64_t* bytes)
         oc::Session* s = S(session);
         s->validate();
         *bytes = oc::OctreeBuilder::estimateFootprint(s->cloudInfo(), s->processingInfo(), s->hostCloud() != nullptr);
+    });
+}
+
+int ocpi_compute_bounding_box(const uint8_t* devicePointCloud, uint32_t pointAmount, uint32_t dataStride,
+                              uint8_t cloudType, int device, double* realBB, double* cubicBB)
+{
+    return execute([&] {
+        if (!realBB || !cubicBB)
+            throw std::runtime_error("bounding box output pointers are null");
+        oc::realBoundingBox(devicePointCloud, pointAmount, dataStride, cloudType, device, realBB);
+        oc::cubicBoundingBox(realBB, cubicBB);
     });
 }

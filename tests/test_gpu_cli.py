@@ -1,0 +1,58 @@
+"""The command-line caller and the GPU bounding box on the MI355X: the box equals the reference
+CLI's host formula exactly, and a CLI run writes the hierarchy the oracle predicts."""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_binding as O
+from hip_memory import DeviceBuffer
+from octree_amd import api
+from octree_amd import workloads as W
+from parity import synthetic
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(REPO, "octree-cuda_amd", "bin", "octree-amd")
+
+
+@pytest.mark.parametrize("cloud_type,n", [(0, 1), (0, 70_001), (1, 250_000)])
+def test_gpu_bounding_box_equals_host_formula(cloud_type, n):
+    print(f"case bounding box type={cloud_type} n={n}", flush=True)
+    cloud, spec, _ = synthetic(n, cloud_type, box=(30.0, 11.0, 4.0), offset=0.0 if cloud_type == 0 else 3e5)
+    real_h, cubic_h = W.bounding_box(cloud, n, spec.stride, cloud_type)
+    buf = DeviceBuffer(cloud)
+    try:
+        real = (ctypes.c_double * 6)()
+        cubic = (ctypes.c_double * 6)()
+        rc = api.library().ocpi_compute_bounding_box(buf.ptr, n, spec.stride, cloud_type, 0, real, cubic)
+        assert rc == 0, api.last_error()
+        assert list(real) == real_h
+        assert list(cubic) == cubic_h
+    finally:
+        buf.close()
+
+
+def test_cli_writes_the_oracle_hierarchy(tmp_path):
+    print("case cli 120000 points f64 inter", flush=True)
+    cloud, spec, cubic = synthetic(120_000, 1, box=(25.0, 25.0, 6.0), offset=1e5)
+    path = tmp_path / "cloud.bin"
+    cloud.tofile(path)
+    out = tmp_path / "potree"
+    r = subprocess.run([CLI, "-f", str(path), "-o", str(out), "-p", str(spec.points), "-t", "double", "-d",
+                        f"27,{spec.scale}", "-g", "64,32", "-m", "2000", "-e", "3.0", "-i"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    build = W.BuildSpec(64, 2000, 3.0, 32, inter=True)
+    orc = O.build(cloud, spec.points, 27, 1, cubic, [float(np.float32(spec.scale))] * 3, build)
+    assert orc.status == 0
+    assert (out / "hierarchy.bin").read_bytes() == orc.hierarchy
+    meta = json.loads((out / "metadata.json").read_text())
+    assert meta["points"] > 0
+    assert os.path.getsize(out / "octree.bin") == int(np.float32(spec.points) * np.float32(3.0)) * 18
+    for name in ("statistics.json", "point_distribution.html", "memory_report.html"):
+        assert (out / name).exists(), name
