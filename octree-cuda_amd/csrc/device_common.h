@@ -122,6 +122,56 @@ __device__ __forceinline__ OutRecord loadRecord(const OutRecord* src)
     return r;
 }
 
+// Wave-level runs: consecutive lanes holding equal keys (inputs arrive spatially sorted, so lanes
+// often share a bitmap word or a cell). Every lane must take part; a lane without work passes a key
+// of its own. start / end are the first and last lane of the lane's run.
+__device__ __forceinline__ uint64_t shflUp64(uint64_t v, int delta)
+{
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), delta, 64);
+    const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), delta, 64);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int lane)
+{
+    const uint32_t lo = __shfl(static_cast<uint32_t>(v), lane, 64);
+    const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), lane, 64);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+struct WaveRun
+{
+    int lane, start, end;
+};
+
+__device__ __forceinline__ WaveRun waveRun(uint64_t key)
+{
+    WaveRun r;
+    r.lane                 = static_cast<int>(threadIdx.x & 63);
+    const uint64_t prev    = shflUp64(key, 1);
+    const bool head        = r.lane == 0 || prev != key;
+    const uint64_t heads   = __ballot(head);
+    const uint64_t upToMe  = (r.lane == 63) ? ~0ull : ((2ull << r.lane) - 1);
+    r.start                = 63 - __clzll(heads & upToMe);
+    const uint64_t after   = heads & ~upToMe;
+    r.end                  = after ? (__ffsll(static_cast<long long>(after)) - 2) : 63;
+    return r;
+}
+
+// Inclusive scan with op over the lanes of each run (log-step, limited to the run).
+template <typename V, typename Op>
+__device__ __forceinline__ V runScan(V v, const WaveRun& r, Op op)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const V n = static_cast<V>(shflUp64(static_cast<uint64_t>(v), o));
+        if (r.lane - o >= r.start)
+            v = op(v, n);
+    }
+    return v;
+}
+
 // grid-stride helpers
 __device__ __forceinline__ uint64_t globalThread()
 {

@@ -231,24 +231,36 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
                                                uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
                                                uint32_t* __restrict__ err)
 {
-    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
-    if (!in.valid)
-        return;
-    const NodeGrid ng = nodeGrid[in.k];
-    T px, py, pz;
-    ps.xyz(in.pos, px, py, pz);
-    uint64_t ix, iy, iz;
-    gridCoords(static_cast<double>(px), static_cast<double>(py), static_cast<double>(pz), ng.grid, ix, iy, iz);
-    const uint32_t gs = ng.grid.gridSize;
-    if (ix >= gs || iy >= gs || iz >= gs)
+    // All lanes take part in the wave-level merge of bit sets bound for the same bitmap word.
+    InputRef in   = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    bool valid    = in.valid;
+    uint32_t cell = ~0u;
+    if (valid)
     {
-        atomicOr(err, ERR_OUTSIDE);
-        cellOf[in.q] = ~0u;  // skipped by the later passes; the build reports the error
-        return;
+        const NodeGrid ng = nodeGrid[in.k];
+        T px, py, pz;
+        ps.xyz(in.pos, px, py, pz);
+        uint64_t ix, iy, iz;
+        gridCoords(static_cast<double>(px), static_cast<double>(py), static_cast<double>(pz), ng.grid, ix, iy, iz);
+        const uint32_t gs = ng.grid.gridSize;
+        if (ix >= gs || iy >= gs || iz >= gs)
+        {
+            atomicOr(err, ERR_OUTSIDE);  // skipped by the later passes; the build reports the error
+            cellOf[in.q] = ~0u;
+            valid        = false;
+        }
+        else
+        {
+            cell         = static_cast<uint32_t>(ix + iy * gs + iz * gs * gs);
+            cellOf[in.q] = cell;
+        }
     }
-    const uint32_t cell = static_cast<uint32_t>(ix + iy * gs + iz * gs * gs);
-    atomicOr(bitmap + (in.k - nodeBase) * wordsPerNode + (cell >> 5), 1u << (cell & 31));
-    cellOf[in.q] = cell;
+    const uint64_t word = valid ? (in.k - nodeBase) * wordsPerNode + (cell >> 5) : ~0ull - threadIdx.x;
+    const WaveRun run   = waveRun(word);
+    const uint32_t bits = runScan<uint32_t>(valid ? 1u << (cell & 31) : 0u, run,
+                                            [](uint32_t x, uint32_t y) { return x | y; });
+    if (valid && run.lane == run.end)
+        atomicOr(bitmap + word, bits);
 }
 
 struct PopcountIn
@@ -317,33 +329,56 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
                                                      uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
                                                      uint32_t* __restrict__ compactOf, uint32_t* __restrict__ rankOf)
 {
-    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
-    if (!in.valid)
-        return;
-    const uint32_t cell = cellOf[in.q];
-    if (cell == ~0u)
-        return;  // the point lay outside its node's grid (reported by kMark)
-    const uint64_t word = (in.k - nodeBase) * wordsPerNode + (cell >> 5);
-    const uint32_t c = cellRank(bitmap, prefix, word, cell & 31);
-    // every point of the cell stores the same key and node
-    __hip_atomic_store(cellKey + c, cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(cellNode + c, in.k - nodeBase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // All lanes take part: runs of lanes in the same cell make one atomic per run.
+    InputRef in         = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    const uint32_t cell = in.valid ? cellOf[in.q] : ~0u;
+    const bool valid    = cell != ~0u;  // an invalid cell: the point lay outside its node's grid
+    uint32_t c          = 0;
+    if (valid)
+    {
+        const uint64_t word = (in.k - nodeBase) * wordsPerNode + (cell >> 5);
+        c                   = cellRank(bitmap, prefix, word, cell & 31);
+    }
+    const WaveRun run = waveRun(valid ? c : ~0ull - threadIdx.x);
+    if (valid && run.lane == run.start)
+    {
+        cellKey[c]  = cell;
+        cellNode[c] = in.k - nodeBase;
+    }
     if (MODE == COLOR_INTER_WEIGHTED)
     {
         // the winner is taken from the cell's payload run in the emit
-        compactOf[in.q] = c;
-        rankOf[in.q]    = atomicAdd(cellCnt + c, 1u);
+        uint32_t base = 0;
+        if (valid && run.lane == run.end)
+            base = atomicAdd(cellCnt + c, static_cast<uint32_t>(run.end - run.start + 1));
+        base = __shfl(base, run.end, 64);
+        if (valid)
+        {
+            compactOf[in.q] = c;
+            rankOf[in.q]    = base + static_cast<uint32_t>(run.lane - run.start);
+        }
         return;
     }
-    const uint32_t idx  = ps.index(in.pos);
-    const uint32_t rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
-    atomicMin(reinterpret_cast<unsigned long long*>(win + c),
-              static_cast<unsigned long long>((static_cast<uint64_t>(rank) << 32) | in.pos));
-    if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
+    uint64_t key = ~0ull, col = 0;
+    if (valid)
     {
-        OutRecord r = ps.record(in.pos);
-        atomicAdd(reinterpret_cast<unsigned long long*>(color + c),
-                  static_cast<unsigned long long>(encodeColors(r.r, r.g, r.b)));
+        const uint32_t idx  = ps.index(in.pos);
+        const uint32_t rank = RANDOM ? fmix32(idx ^ nodeGrid[in.k].salt) : idx;
+        key                 = (static_cast<uint64_t>(rank) << 32) | in.pos;
+        if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
+        {
+            const OutRecord r = ps.record(in.pos);
+            col               = encodeColors(r.r, r.g, r.b);
+        }
+    }
+    key = runScan<uint64_t>(key, run, [](uint64_t x, uint64_t y) { return x < y ? x : y; });
+    if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
+        col = runScan<uint64_t>(col, run, [](uint64_t x, uint64_t y) { return x + y; });
+    if (valid && run.lane == run.end)
+    {
+        atomicMin(reinterpret_cast<unsigned long long*>(win + c), static_cast<unsigned long long>(key));
+        if (MODE == COLOR_INTRA || MODE == COLOR_INTER)
+            atomicAdd(reinterpret_cast<unsigned long long*>(color + c), static_cast<unsigned long long>(col));
     }
 }
 
