@@ -146,7 +146,7 @@ private:
             itsNodeSorted, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
 
     // Morton bucket sort of the cloud
-    DeviceBuffer itsBucketed, itsSorted, itsFineCode, itsBucketCount, itsBucketStart, itsBucketCursor;
+    DeviceBuffer itsBucketed, itsSorted, itsFineCode, itsPerm, itsBucketCount, itsBucketStart, itsBucketCursor;
 
     // output region [0, E) (lut: point index, out: 18-byte record): leaf points at [0, N), internal
     // subsamples at [N, N + I) after the final placement. Staging (st*): subsampled points in

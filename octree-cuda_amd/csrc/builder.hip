@@ -192,6 +192,7 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     itsSorted.ensure(sortedBytes * cloud.pointAmount);
     itsBucketed.ensure(sortedBytes * cloud.pointAmount);
     itsFineCode.ensure(sizeof(uint16_t) * cloud.pointAmount);
+    itsPerm.ensure(sizeof(uint32_t) * cloud.pointAmount);
     itsBucketCount.ensure(sizeof(uint32_t) * (itsBuckets + 1));
     itsBucketStart.ensure(sizeof(uint32_t) * (itsBuckets + 1));
     itsBucketCursor.ensure(sizeof(uint32_t) * (itsBuckets + 1));
@@ -210,6 +211,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
     return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsNodeKey,      &itsNodeCount,   &itsNodePoints,
             &itsNodeData,    &itsNodeParent,  &itsNodeChild,     &itsNodeFlags,    &itsNodeSorted,  &itsNodeAux,
             &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
+            &itsPerm,
             &itsBucketCount, &itsBucketStart, &itsBucketCursor,  &itsLut,          &itsOut,         &itsStageNode,
             &itsStLut,       &itsStOut,       &itsStPts,         &itsInternalList, &itsSegCount,    &itsSegSrc,
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
@@ -252,7 +254,7 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     bytes += 8 * dense;                                     // count pyramid, dense -> sparse
     bytes += 8 * mBound + 69 * mBound;                      // node keys / counts, node arrays
     bytes += 22 * e;                                        // lut + output records
-    bytes += (2 * sortedPointBytes(cloud) + 2) * n;         // bucketed, sorted, fine codes
+    bytes += (2 * sortedPointBytes(cloud) + 2 + 4) * n;     // bucketed, sorted, fine codes, permutation
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
     // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
@@ -402,9 +404,12 @@ void OctreeBuilder::countPoints()
         fp.levelOffset[l] = itsLevelOffset[l];
         fp.levelGrid[l]   = itsLevelGrid[l];
     }
-    kFineSort<T><<<nb, SORT_THREADS, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(), itsFineCode.as<uint16_t>(),
-                                                     bucketStart, fp, itsCount.as<uint32_t>(),
-                                                     itsSorted.as<SortedPoint<T>>());
+    kFineSort<<<nb, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart, fp,
+                                                  itsCount.as<uint32_t>(), itsPerm.as<uint32_t>());
+    OC_LAUNCH_CHECK();
+    kGatherSorted<T><<<gridFor(n, 16384), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
+                                                                 itsPerm.as<uint32_t>(), n,
+                                                                 itsSorted.as<SortedPoint<T>>());
     OC_LAUNCH_CHECK();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));
 }

@@ -12,7 +12,8 @@
 //                   colour) plus the 3*lc-bit fine code as contiguous runs
 //   kFineSort       one workgroup per bucket: LDS histogram over the bucket's (2^lc)^3 level-0
 //                   cells, exclusive scan, dense counts of levels 0..lc of the count pyramid,
-//                   then a scatter into full Morton order
+//                   then the permutation into full Morton order (4-byte indices)
+//   kGatherSorted   the records gathered through the permutation, written in order
 // In Morton order every octree node is one contiguous range of the sorted cloud, so after the
 // merge a leaf's points are the range [sortedStart, sortedStart + count) and the distribution is
 // a coalesced copy of those ranges into the leaf data blocks (kDistributeSorted).
@@ -279,12 +280,9 @@ struct FineSortParams
     uint32_t levelGrid[MAX_LEVELS];
 };
 
-template <typename T>
-__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const SortedPoint<T>* __restrict__ src,
-                                                           const uint16_t* __restrict__ fineCode,
+__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
                                                            const uint32_t* __restrict__ bucketStart, FineSortParams fp,
-                                                           uint32_t* __restrict__ count,
-                                                           SortedPoint<T>* __restrict__ dst)
+                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
 {
     __shared__ uint32_t h[MAX_FINE_CELLS + 1];
     __shared__ uint32_t scratch[SORT_WAVES];
@@ -335,11 +333,22 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const SortedPoint<T>* 
     }
     __syncthreads();
 
+    // the permutation into Morton order: 4-byte scattered stores instead of 32-byte records
     for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
     {
         const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
-        dst[lo + slot]      = src[i];
+        perm[lo + slot]     = i;
     }
+}
+
+// The sorted cloud: gathered records, written in order.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void kGatherSorted(const SortedPoint<T>* __restrict__ bucketed,
+                                                        const uint32_t* __restrict__ perm, uint32_t n,
+                                                        SortedPoint<T>* __restrict__ sorted)
+{
+    for (uint64_t j = globalThread(); j < n; j += globalStride())
+        sorted[j] = bucketed[perm[j]];
 }
 
 // Start of every leaf's range in the sorted cloud: the points of all nodes that precede it in
