@@ -258,8 +258,7 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
     // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
-    uint64_t span = proc.subsamplingGrid;
-    const uint64_t wordsPerNode = (span * span * span + 31) / 32;
+    const uint64_t wordsPerNode = (rowsPerNode(proc.subsamplingGrid) * proc.subsamplingGrid + 31) / 32;
     const uint64_t levelNodes   = std::min<uint64_t>(mBound, n / std::max<uint32_t>(1, proc.mergingThreshold) + 8);
     bytes += (12 + sizeof(Payload<double, true>)) * e + 32 * staged;
     bytes += std::min<uint64_t>(kBitmapBudget, 8 * wordsPerNode * levelNodes);
@@ -559,7 +558,7 @@ void OctreeBuilder::subsampleLevels()
     const uint32_t N = itsCloud.pointAmount;
     const uint32_t g = itsProc.subsamplingGrid;
     const double cubicSize   = itsCloud.bbMax[0] - itsCloud.bbMin[0];  // metadata.cuh:66-69
-    const uint64_t cellsPerNode = static_cast<uint64_t>(g) * g * g;
+    const uint64_t cellsPerNode = rowsPerNode(g) * g;
     const uint64_t wordsPerNode = (cellsPerNode + 31) / 32;
     const uint32_t batchNodes   = static_cast<uint32_t>(
             std::max<uint64_t>(1, kBitmapBudget / (2 * sizeof(uint32_t) * wordsPerNode)));
@@ -751,7 +750,8 @@ void OctreeBuilder::subsampleLevels()
             OC_HIP_CHECK(hipEventRecord(ev[2], itsStream));
             if (weighted)
             {
-                const uint32_t emitBlocks = static_cast<uint32_t>((cells + BLOCK - 1) / BLOCK);
+                const uint32_t emitBlocks = static_cast<uint32_t>(((cells + BLOCK - 1) / BLOCK + 8 * EMIT_CHUNK - 1) /
+                                                                  (8 * EMIT_CHUNK) * (8 * EMIT_CHUNK));
                 if (itsProc.useRandomSubsampling)
                     kEmitWeighted<T, true><<<emitBlocks, BLOCK, 0, itsStream>>>(
                             cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,

@@ -3,7 +3,8 @@
 // All internal nodes of one octree level are processed together (in memory-bounded batches of
 // nodes). Each node of a batch owns an occupancy bitmap over its g^3 subsampling grid and the
 // exclusive popcount prefix of that bitmap's words, i.e. a rank dictionary: the occupied cells of
-// the batch get dense indices in (node, ascending cell) order. Per batch:
+// the batch get dense indices in (node, cell index) order; the cell index groups the grid's x-rows
+// into 4 x 4 pencils (rowIndex). Per batch:
 //   kMark        input point -> cell (the reference's mapPointToGrid with the node's gridding),
 //                set the cell's bit
 //   prefix scan  popcounts of all bitmap words -> dense index of every occupied cell
@@ -25,6 +26,37 @@
 namespace oc {
 
 constexpr uint32_t TILE = 256;  // inputs per workgroup in the per-input kernels
+
+// Cell index of a node grid. The x-rows (fixed y, z) are grouped into 4 x 4 pencils, pencils in
+// (z / 4, y / 4) order and rows inside a pencil in (z % 4, y % 4) order; a cell is row * g + x.
+// A row stays one contiguous run of bits (and of dense ranks), and the 9 rows a cell's 3x3x3
+// neighbourhood touches lie a few rows apart instead of a whole z-plane apart.
+constexpr uint32_t PENCIL = 4;
+constexpr uint32_t EMIT_CHUNK = 32;  // consecutive cell blocks of the weighted emit per XCD turn
+
+__host__ __device__ __forceinline__ uint32_t pencilsPerAxis(uint32_t g)
+{
+    return (g + PENCIL - 1) / PENCIL;
+}
+
+__host__ __device__ __forceinline__ uint64_t rowsPerNode(uint32_t g)
+{
+    const uint64_t p = pencilsPerAxis(g);
+    return p * p * PENCIL * PENCIL;
+}
+
+__host__ __device__ __forceinline__ uint32_t rowIndex(uint32_t y, uint32_t z, uint32_t g)
+{
+    const uint32_t p = pencilsPerAxis(g);
+    return (((z / PENCIL) * p + (y / PENCIL)) * PENCIL + (z % PENCIL)) * PENCIL + (y % PENCIL);
+}
+
+__host__ __device__ __forceinline__ void rowCoords(uint32_t row, uint32_t g, uint32_t& y, uint32_t& z)
+{
+    const uint32_t p = pencilsPerAxis(g), pencil = row / (PENCIL * PENCIL), inside = row % (PENCIL * PENCIL);
+    y = (pencil % p) * PENCIL + inside % PENCIL;
+    z = (pencil / p) * PENCIL + inside / PENCIL;
+}
 
 // The points a subsampling level reads: leaf points at positions [0, N) of the Morton-sorted
 // cloud (a leaf's points are one contiguous range there) and already subsampled points of lower
@@ -251,7 +283,7 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
         }
         else
         {
-            cell         = static_cast<uint32_t>(ix + iy * gs + iz * gs * gs);
+            cell = rowIndex(static_cast<uint32_t>(iy), static_cast<uint32_t>(iz), gs) * gs + static_cast<uint32_t>(ix);
             cellOf[in.q] = cell;
         }
     }
@@ -462,8 +494,10 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
             const uint32_t key    = cellKey[c];
             const uint32_t gs     = ng.grid.gridSize;
             const uint64_t words  = static_cast<uint64_t>(kl) * wordsPerNode;
-            const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>((key / gs) % gs),
-                          cz = static_cast<int32_t>(key / (gs * gs));
+            uint32_t ry, rz;
+            rowCoords(key / gs, gs, ry, rz);
+            const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>(ry),
+                          cz = static_cast<int32_t>(rz);
             const int32_t x0 = cx > 0 ? cx - 1 : 0;
             const int32_t x1 = cx + 1 < static_cast<int32_t>(gs) ? cx + 1 : cx;
             const uint32_t span = static_cast<uint32_t>(x1 - x0 + 1);
@@ -474,8 +508,8 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
                 const int32_t ny = cy + (r / 3) - 1, nz = cz + (r % 3) - 1;
                 if (ny < 0 || nz < 0 || ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
                     continue;
-                const uint64_t L0 = static_cast<uint64_t>(x0) + static_cast<uint64_t>(ny) * gs +
-                                    static_cast<uint64_t>(nz) * gs * gs;
+                const uint64_t L0 = static_cast<uint64_t>(x0) +
+                                    static_cast<uint64_t>(rowIndex(static_cast<uint32_t>(ny), static_cast<uint32_t>(nz), gs)) * gs;
                 const uint64_t w0 = words + (L0 >> 5);
                 const uint32_t b0 = static_cast<uint32_t>(L0 & 31);
                 uint64_t v        = bitmap[w0];
@@ -527,7 +561,11 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
                                                         const Payload<T, RANDOM>* __restrict__ payload,
                                                         PointStore<T> ps, uint32_t* __restrict__ stageNode)
 {
-    const uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
+    const uint32_t b     = blockIdx.x;
+    const uint32_t round = (b / 8) / EMIT_CHUNK;
+    const uint32_t block = round * 8 * EMIT_CHUNK + (b % 8) * EMIT_CHUNK + (b / 8) % EMIT_CHUNK;
+    const uint32_t c     = block * BLOCK + threadIdx.x;
     if (c >= cells)
         return;
     const uint32_t kl    = cellNode[c];
@@ -535,8 +573,9 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
     const uint32_t key   = cellKey[c];
     const uint32_t gs    = ng.grid.gridSize;
     const uint64_t words = static_cast<uint64_t>(kl) * wordsPerNode;
-    const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>((key / gs) % gs),
-                  cz = static_cast<int32_t>(key / (gs * gs));
+    uint32_t ry, rz;
+    rowCoords(key / gs, gs, ry, rz);
+    const int32_t cx = static_cast<int32_t>(key % gs), cy = static_cast<int32_t>(ry), cz = static_cast<int32_t>(rz);
     const int32_t x0 = cx > 0 ? cx - 1 : 0;
     const int32_t x1 = cx + 1 < static_cast<int32_t>(gs) ? cx + 1 : cx;
     const uint32_t span = static_cast<uint32_t>(x1 - x0 + 1);
@@ -551,8 +590,8 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
         first[r] = 0;
         if (ny < 0 || nz < 0 || ny >= static_cast<int32_t>(gs) || nz >= static_cast<int32_t>(gs))
             continue;
-        const uint64_t L0 =
-                static_cast<uint64_t>(x0) + static_cast<uint64_t>(ny) * gs + static_cast<uint64_t>(nz) * gs * gs;
+        const uint64_t L0 = static_cast<uint64_t>(x0) +
+                            static_cast<uint64_t>(rowIndex(static_cast<uint32_t>(ny), static_cast<uint32_t>(nz), gs)) * gs;
         const uint64_t w0 = words + (L0 >> 5);
         const uint32_t b0 = static_cast<uint32_t>(L0 & 31);
         uint64_t v        = bitmap[w0];
