@@ -3,9 +3,11 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "pinned_pool.h"
+
 namespace oc {
 
-std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& source)
+std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& source, PinnedPool* pool)
 {
     auto r       = std::unique_ptr<HostBuildResult>(new HostBuildResult());
     r->itsCloud  = source.cloud();
@@ -17,12 +19,19 @@ std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& sou
     r->itsUsed   = source.used();
     r->itsTimings = source.stageTimings();
     source.copyNodes(r->itsNodes);
-    r->itsLut.resize(r->itsUsed);
-    r->itsOut.resize(r->itsUsed * sizeof(OutRecord));
+    const size_t lutBytes = r->itsUsed * sizeof(uint32_t);
+    const size_t bytes    = lutBytes + r->itsUsed * sizeof(OutRecord) + 1;
+    if (pool)
+        r->itsBlock = pool->acquire(bytes);
+    else
+        r->itsBlock = std::shared_ptr<uint8_t>(new uint8_t[bytes], std::default_delete<uint8_t[]>());
+    uint8_t* base = r->itsBlock.get();
+    r->itsLut     = reinterpret_cast<const uint32_t*>(base);
+    r->itsOut     = base + lutBytes;
     if (r->itsUsed)
     {
-        source.copyLut(r->itsLut.data(), r->itsUsed);
-        source.copyOutputRange(r->itsOut.data(), 0, r->itsUsed);
+        source.copyLut(reinterpret_cast<uint32_t*>(base), r->itsUsed);
+        source.copyOutputRange(base + lutBytes, 0, r->itsUsed);
     }
     return r;
 }
@@ -33,7 +42,7 @@ void HostBuildResult::copyLut(uint32_t* dst, uint64_t count) const
         throw std::runtime_error("copy beyond the output capacity");
     const uint64_t head = count < itsUsed ? count : itsUsed;
     if (head)
-        std::memcpy(dst, itsLut.data(), head * sizeof(uint32_t));
+        std::memcpy(dst, itsLut, head * sizeof(uint32_t));
     if (count > head)
         std::memset(dst + head, 0, (count - head) * sizeof(uint32_t));
 }
@@ -46,7 +55,7 @@ void HostBuildResult::copyOutputRange(uint8_t* dst, uint64_t first, uint64_t cou
     if (first < itsUsed)
     {
         copied = (first + count <= itsUsed ? count : itsUsed - first);
-        std::memcpy(dst, itsOut.data() + first * sizeof(OutRecord), copied * sizeof(OutRecord));
+        std::memcpy(dst, itsOut + first * sizeof(OutRecord), copied * sizeof(OutRecord));
     }
     if (count > copied)
         std::memset(dst + copied * sizeof(OutRecord), 0, (count - copied) * sizeof(OutRecord));

@@ -70,7 +70,9 @@ public:
 class HostBuildResult final : public BuildResult
 {
 public:
-    static std::unique_ptr<HostBuildResult> capture(const BuildResult& source);
+    // Copies the source's nodes and the used part of its LUT and output; with a pool, the LUT
+    // and output land in one page-locked buffer from it.
+    static std::unique_ptr<HostBuildResult> capture(const BuildResult& source, class PinnedPool* pool = nullptr);
 
     const CloudInfo& cloud() const override { return itsCloud; }
     const ProcessingInfo& processing() const override { return itsProc; }
@@ -92,8 +94,9 @@ private:
     uint64_t itsE = 0, itsUsed = 0;
     std::vector<float> itsTimings;
     HostOctree itsNodes;
-    std::vector<uint32_t> itsLut;
-    std::vector<uint8_t> itsOut;
+    std::shared_ptr<uint8_t> itsBlock;  // the LUT (used entries) followed by the output records
+    const uint32_t* itsLut = nullptr;
+    const uint8_t* itsOut  = nullptr;
 };
 
 }  // namespace oc

@@ -31,6 +31,7 @@ BuildService::BuildService(int device, uint32_t slots, uint64_t memoryBudget)
 {
     if (slots == 0 || slots > 64)
         throw std::runtime_error("a build service needs 1 to 64 slots");
+    itsPinned = std::make_shared<PinnedPool>(size_t(16) << 30);
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
         throw std::runtime_error("device " + std::to_string(device) + " does not exist");
@@ -212,7 +213,7 @@ void BuildService::run(uint32_t index)
             slot.builder->setup(session->cloudInfo(), session->processingInfo(), session->hostCloud(),
                                 session->deviceCloud());
             slot.builder->generate();
-            result = HostBuildResult::capture(*slot.builder);
+            result = HostBuildResult::capture(*slot.builder, itsPinned.get());
         }
         catch (const std::exception& e)
         {

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "builder.h"
+#include "pinned_pool.h"
 #include "session.h"
 
 namespace oc {
@@ -70,6 +71,7 @@ private:
     uint64_t itsBudget;
     uint32_t itsActive;
     std::vector<std::unique_ptr<Slot>> itsSlots;
+    std::shared_ptr<PinnedPool> itsPinned;  // results are copied into page-locked buffers
     std::deque<Session*> itsQueue;
     std::mutex itsMutex;
     std::condition_variable itsWork;  // slots wait for requests or memory
