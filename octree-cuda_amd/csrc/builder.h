@@ -72,6 +72,9 @@ public:
     void setup(const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
                const uint8_t* deviceCloud);
     void generate();
+    // Per-batch subsampling events (the insert / emit split of the stage timings); off, a build
+    // makes fewer runtime calls, which matters when many builders share one process.
+    void setDetailedTimings(bool on) { itsDetailedTimings = on; }
 
     // Device bytes held by the arena, and its release (the next setup grows it again).
     uint64_t arenaBytes() const;
@@ -115,6 +118,7 @@ private:
     int itsDevice;
     hipStream_t itsStream = nullptr;
     bool itsReady         = false;  // setup() has run
+    bool itsDetailedTimings = true;
     CloudInfo itsCloud;
     ProcessingInfo itsProc;
     ColorMode itsColorMode = COLOR_NONE;
@@ -177,7 +181,7 @@ private:
         EV_SLOTS
     };
     hipEvent_t itsEvents[EV_SLOTS] = {};
-    // per subsampling batch: evaluate begin/end, emit begin/end
+    // per subsampling batch: evaluate begin/end, emit begin/end (when detailed timings are on)
     std::vector<std::array<hipEvent_t, 4>> itsBatchEvents;
     uint32_t itsBatchesRun = 0;
     std::array<hipEvent_t, 4>& batchEvents();

@@ -657,11 +657,13 @@ void OctreeBuilder::subsampleLevels()
             itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(words) + 1));
             uint32_t* bitmap = itsBitmap.as<uint32_t>();
             uint32_t* prefix = itsPrefix.as<uint32_t>();
-            auto& ev         = batchEvents();
+            std::array<hipEvent_t, 4> none{};
+            auto& ev         = itsDetailedTimings ? batchEvents() : none;
 
             kFill32<<<gridFor(words), BLOCK, 0, itsStream>>>(bitmap, words, 0);
             OC_LAUNCH_CHECK();
-            OC_HIP_CHECK(hipEventRecord(ev[0], itsStream));
+            if (itsDetailedTimings)
+                OC_HIP_CHECK(hipEventRecord(ev[0], itsStream));
             if (blocks > 0)
             {
                 kMark<T><<<blocks, TILE, 0, itsStream>>>(
@@ -719,7 +721,8 @@ void OctreeBuilder::subsampleLevels()
                 }
                 OC_LAUNCH_CHECK();
             }
-            OC_HIP_CHECK(hipEventRecord(ev[1], itsStream));
+            if (itsDetailedTimings)
+                OC_HIP_CHECK(hipEventRecord(ev[1], itsStream));
 
             if (weighted)
             {
@@ -747,7 +750,8 @@ void OctreeBuilder::subsampleLevels()
                 }
             }
 
-            OC_HIP_CHECK(hipEventRecord(ev[2], itsStream));
+            if (itsDetailedTimings)
+                OC_HIP_CHECK(hipEventRecord(ev[2], itsStream));
             if (weighted)
             {
                 const uint32_t emitBlocks = static_cast<uint32_t>(((cells + BLOCK - 1) / BLOCK + 8 * EMIT_CHUNK - 1) /
@@ -781,7 +785,8 @@ void OctreeBuilder::subsampleLevels()
                 }
                 OC_LAUNCH_CHECK();
             }
-            OC_HIP_CHECK(hipEventRecord(ev[3], itsStream));
+            if (itsDetailedTimings)
+                OC_HIP_CHECK(hipEventRecord(ev[3], itsStream));
             cursor += cells;
         }
     }

@@ -5,7 +5,9 @@
 // its tiles with the tile offset and hands (index, exclusive prefix, value) to an output functor.
 // Inside a workgroup the scan is a wave64 shuffle scan (6 steps) plus one LDS step across the
 // four waves. Inputs and outputs are functors so the flags / values are generated and consumed in
-// the same pass as the scan (e.g. occupancy flags straight from the counting grid).
+// the same pass as the scan (e.g. occupancy flags straight from the counting grid). Inputs of at
+// most SCAN_SINGLE_TILES tiles take one launch of a single workgroup instead (small builds make
+// many small scans, and every launch is a runtime call).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -133,21 +135,55 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64
     }
 }
 
+// Small inputs: one workgroup walks the tiles in order with a running carry (one launch).
+template <typename T, typename In, typename Out>
+__global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
+{
+    __shared__ T lds[SCAN_WAVES];
+    T carry = 0;
+    for (uint64_t tileBase = 0; tileBase < n; tileBase += SCAN_TILE)
+    {
+        const uint64_t base = tileBase + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
+        T v[SCAN_ITEMS];
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            v[k] = (base + k < n) ? in(base + k) : T(0);
+            s += v[k];
+        }
+        T chunk;
+        T off = blockExclusiveScan(s, chunk, lds) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            if (base + k < n)
+                out(base + k, off, v[k]);
+            off += v[k];
+        }
+        carry += chunk;
+    }
+    if (threadIdx.x == 0 && total)
+        *total = carry;
+}
+
 inline uint64_t scanTiles(uint64_t n)
 {
     return (n + SCAN_TILE - 1) / SCAN_TILE;
 }
 
+// Inputs up to this many tiles are scanned by kScanSingle.
+constexpr uint64_t SCAN_SINGLE_TILES = 16;
+
 // Exclusive scan of in(0..n-1); out(i, prefix, value) for each i; *total (device) = sum.
-// tileScratch must hold scanTiles(n) values.
+// tileScratch must hold scanTiles(n) values (unused for inputs of at most SCAN_SINGLE_TILES tiles).
 template <typename T, typename In, typename Out>
 void scanExclusive(uint64_t n, In in, Out out, T* tileScratch, T* total, hipStream_t stream)
 {
     uint64_t tiles = scanTiles(n);
-    if (tiles == 0)
+    if (tiles <= SCAN_SINGLE_TILES)
     {
-        if (total)
-            (void)hipMemsetAsync(total, 0, sizeof(T), stream);
+        kScanSingle<T, In, Out><<<1, SCAN_THREADS, 0, stream>>>(in, out, n, total);
         return;
     }
     unsigned grid = static_cast<unsigned>(tiles < 8192 ? tiles : 8192);

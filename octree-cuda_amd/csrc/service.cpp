@@ -39,6 +39,7 @@ BuildService::BuildService(int device, uint32_t slots, uint64_t memoryBudget)
     {
         auto slot     = std::make_unique<Slot>();
         slot->builder = std::make_unique<OctreeBuilder>(device);
+        slot->builder->setDetailedTimings(false);
         itsSlots.push_back(std::move(slot));
     }
     for (uint32_t i = 0; i < slots; ++i)
