@@ -260,7 +260,7 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
     const uint64_t wordsPerNode = (rowsPerNode(proc.subsamplingGrid) * proc.subsamplingGrid + 31) / 32;
     const uint64_t levelNodes   = std::min<uint64_t>(mBound, n / std::max<uint32_t>(1, proc.mergingThreshold) + 8);
-    bytes += (12 + sizeof(Payload<double, true>)) * e + 32 * staged;
+    bytes += (12 + sizeof(WeightPayload) + sizeof(uint64_t)) * e + 32 * staged;
     bytes += std::min<uint64_t>(kBitmapBudget, 8 * wordsPerNode * levelNodes);
     return bytes;
 }
@@ -406,7 +406,7 @@ void OctreeBuilder::countPoints()
     kFineSort<<<nb, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart, fp,
                                                   itsCount.as<uint32_t>(), itsPerm.as<uint32_t>());
     OC_LAUNCH_CHECK();
-    kGatherSorted<T><<<gridFor(n, 16384), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
+    kGatherSorted<T><<<gatherGrid(n), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
                                                                  itsPerm.as<uint32_t>(), n,
                                                                  itsSorted.as<SortedPoint<T>>());
     OC_LAUNCH_CHECK();
@@ -634,8 +634,10 @@ void OctreeBuilder::subsampleLevels()
         {
             itsCompactOf.ensure(sizeof(uint32_t) * inputs);
             itsRankOf.ensure(sizeof(uint32_t) * inputs);
-            itsPayload.ensure(sizeof(Payload<T, true>) * inputs);
+            itsPayload.ensure((sizeof(WeightPayload) + sizeof(uint64_t)) * inputs);  // payloads, then winner keys
         }
+        uint64_t* winKey =
+                weighted ? reinterpret_cast<uint64_t*>(itsPayload.as<uint8_t>() + sizeof(WeightPayload) * inputs) : nullptr;
         kTileList<<<gridFor(segments), BLOCK, 0, itsStream>>>(segments, itsSegCount.as<uint32_t>(),
                                                               itsSegTile.as<uint32_t>(), itsTileList.as<uint32_t>());
         OC_LAUNCH_CHECK();
@@ -738,14 +740,14 @@ void OctreeBuilder::subsampleLevels()
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
-                                itsPayload.as<Payload<T, true>>());
+                                itsPayload.as<WeightPayload>(), winKey);
                     else
                         kScatterPayload<T, false><<<blocks, TILE, 0, itsStream>>>(
                                 tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
-                                itsPayload.as<Payload<T, false>>());
+                                itsPayload.as<WeightPayload>(), winKey);
                     OC_LAUNCH_CHECK();
                 }
             }
@@ -756,16 +758,10 @@ void OctreeBuilder::subsampleLevels()
             {
                 const uint32_t emitBlocks = static_cast<uint32_t>(((cells + BLOCK - 1) / BLOCK + 8 * EMIT_CHUNK - 1) /
                                                                   (8 * EMIT_CHUNK) * (8 * EMIT_CHUNK));
-                if (itsProc.useRandomSubsampling)
-                    kEmitWeighted<T, true><<<emitBlocks, BLOCK, 0, itsStream>>>(
-                            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
-                            prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
-                            itsPayload.as<Payload<T, true>>(), ps, itsStageNode.as<uint32_t>());
-                else
-                    kEmitWeighted<T, false><<<emitBlocks, BLOCK, 0, itsStream>>>(
-                            cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
-                            prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
-                            itsPayload.as<Payload<T, false>>(), ps, itsStageNode.as<uint32_t>());
+                kEmitWeighted<T><<<emitBlocks, BLOCK, 0, itsStream>>>(
+                        cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
+                        prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
+                        itsPayload.as<WeightPayload>(), winKey, ps, itsStageNode.as<uint32_t>());
                 OC_LAUNCH_CHECK();
             }
             else

@@ -341,14 +341,37 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
     }
 }
 
-// The sorted cloud: gathered records, written in order.
+// The sorted cloud: gathered records, written in order. The permutation only moves a record
+// inside its bucket, so the reads of one output range hit one bucket's lines. Workgroup b runs on
+// XCD b mod 8, and XCD x gathers the x-th eighth of the output front to back: each bucket's lines
+// are fetched into a single L2 and reused by all of its points instead of by up to eight L2s.
+constexpr int GATHER_ITEMS      = 4;
+constexpr uint32_t GATHER_TILE  = BLOCK * GATHER_ITEMS;
+constexpr uint32_t XCD_COUNT    = 8;
+
+// Workgroups of kGatherSorted for n points (a multiple of XCD_COUNT).
+inline unsigned gatherGrid(uint32_t n)
+{
+    const uint64_t tiles = (static_cast<uint64_t>(n) + GATHER_TILE - 1) / GATHER_TILE;
+    const uint64_t perXcd = (tiles + XCD_COUNT - 1) / XCD_COUNT;
+    return static_cast<unsigned>((perXcd ? perXcd : 1) * XCD_COUNT);
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void kGatherSorted(const SortedPoint<T>* __restrict__ bucketed,
                                                         const uint32_t* __restrict__ perm, uint32_t n,
                                                         SortedPoint<T>* __restrict__ sorted)
 {
-    for (uint64_t j = globalThread(); j < n; j += globalStride())
-        sorted[j] = bucketed[perm[j]];
+    const uint32_t perXcd = gridDim.x / XCD_COUNT;
+    const uint64_t tile   = static_cast<uint64_t>(blockIdx.x % XCD_COUNT) * perXcd + blockIdx.x / XCD_COUNT;
+    const uint64_t base   = tile * GATHER_TILE + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < GATHER_ITEMS; ++k)
+    {
+        const uint64_t j = base + static_cast<uint64_t>(k) * BLOCK;
+        if (j < n)
+            sorted[j] = bucketed[perm[j]];
+    }
 }
 
 // Start of every leaf's range in the sorted cloud: the points of all nodes that precede it in

@@ -414,26 +414,31 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
     }
 }
 
-// Point payload of the weighted filter, stored in cell order so that a cell's points are one
-// contiguous run: coordinates in the cloud's type, the source point index and colour (r | g << 8 |
-// b << 16; weighted averages of 8-bit colours stay below 256), and for the explicit method the
-// point's rank. The winner of a cell is its point of minimum rank (the index itself for the
-// implicit method; both are bijective, so there are no ties).
-template <typename T, bool RANDOM>
-struct alignas(sizeof(T) == 8 ? 16 : 4) Payload
+// Per-point data of the weighted filter, in cell order so that a cell's points are one contiguous
+// run (cellStart[cell] + arrival rank), in two arrays:
+//   WeightPayload: the point's offset from its own cell's centre, (p - centre) rounded to float,
+//     and its colour r | g << 8 | b << 16 with the cell's x coordinate mod 4 in bits 24-25 (a
+//     neighbour's x differs from the selecting cell's by -1, 0 or 1, which x mod 4 identifies).
+//     16 bytes: the select's loop reads one record per lane from lines that differ between lanes,
+//     and every load instruction costs the memory pipeline a pass per line touched.
+//   winner key: rank << 32 | source position, the rank being the point index (implicit method) or
+//     its hash (explicit). Ranks are bijective, so the minimum key of a cell names its winner.
+struct alignas(16) WeightPayload
 {
-    T x, y, z;
-    uint32_t idx, rgb;
-    __device__ __forceinline__ uint32_t rank() const { return idx; }
+    float ox, oy, oz;
+    uint32_t rgbx;
 };
+static_assert(sizeof(WeightPayload) == 16, "weighted payload is one 16-byte load");
 
-template <typename T>
-struct alignas(8) Payload<T, true>
+// Centre of cell (ix, iy, iz) of a node grid, in the select's expression order (min + i t + t / 2).
+__device__ __forceinline__ void cellCentre(const Gridding& g, uint32_t ix, uint32_t iy, uint32_t iz, double& x,
+                                           double& y, double& z)
 {
-    T x, y, z;
-    uint32_t idx, rgb, rnk;
-    __device__ __forceinline__ uint32_t rank() const { return rnk; }
-};
+    const double t = g.bbSize / g.gridSize;
+    x              = g.minX + ix * t + (t * 0.5);
+    y              = g.minY + iy * t + (t * 0.5);
+    z              = g.minZ + iz * t + (t * 0.5);
+}
 
 template <typename T, bool RANDOM>
 __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
@@ -446,19 +451,36 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
                                                          const uint32_t* __restrict__ rankOf,
                                                          const uint32_t* __restrict__ cellStart,
                                                          const uint32_t* __restrict__ cellOf, PointStore<T> ps,
-                                                         Payload<T, RANDOM>* __restrict__ payload)
+                                                         WeightPayload* __restrict__ payload,
+                                                         uint64_t* __restrict__ winKey)
 {
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
-    if (!in.valid || cellOf[in.q] == ~0u)
+    if (!in.valid)
         return;
+    const uint32_t cell = cellOf[in.q];
+    if (cell == ~0u)
+        return;
+    const NodeGrid ng = nodeGrid[in.k];
+    const uint32_t gs = ng.grid.gridSize;
+    uint32_t iy, iz;
+    rowCoords(cell / gs, gs, iy, iz);
+    const uint32_t ix = cell % gs;
+    double cx, cy, cz;
+    cellCentre(ng.grid, ix, iy, iz, cx, cy, cz);
+    T px, py, pz;
+    ps.xyz(in.pos, px, py, pz);
     const OutRecord r = ps.record(in.pos);
-    Payload<T, RANDOM> pl;
-    ps.xyz(in.pos, pl.x, pl.y, pl.z);
-    pl.idx = ps.index(in.pos);
-    pl.rgb = static_cast<uint32_t>(r.r) | (static_cast<uint32_t>(r.g) << 8) | (static_cast<uint32_t>(r.b) << 16);
-    if constexpr (RANDOM)
-        pl.rnk = fmix32(pl.idx ^ nodeGrid[in.k].salt);
-    payload[cellStart[compactOf[in.q]] + rankOf[in.q]] = pl;
+    WeightPayload pl;
+    pl.ox   = static_cast<float>(static_cast<double>(px) - cx);
+    pl.oy   = static_cast<float>(static_cast<double>(py) - cy);
+    pl.oz   = static_cast<float>(static_cast<double>(pz) - cz);
+    pl.rgbx = static_cast<uint32_t>(r.r) | (static_cast<uint32_t>(r.g) << 8) | (static_cast<uint32_t>(r.b) << 16) |
+              ((ix & 3u) << 24);
+    const uint32_t idx  = ps.index(in.pos);
+    const uint32_t rank = RANDOM ? fmix32(idx ^ ng.salt) : idx;
+    const uint32_t slot = cellStart[compactOf[in.q]] + rankOf[in.q];
+    payload[slot]       = pl;
+    winKey[slot]        = (static_cast<uint64_t>(rank) << 32) | in.pos;
 }
 
 // Select (rp_subsampling.cuh:70-321, fp_subsampling.cuh:30-306): one output record per occupied
@@ -547,10 +569,17 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
 // weighted decode rp_subsampling.cuh:222-230): the gathered average over the points of the 3x3x3
 // neighbourhood, with the weight of a point p for this cell c
 //   res = float(sqrt(|centre(c) - p|^2) / diag);  w = expf(-(res * res) * 10)
-// and colour u16(sum c w / sum w). The neighbourhood is walked as 9 x-rows, each one contiguous
-// run of payloads; the winner is the minimum-rank point of the centre cell's own run, whose
-// coordinates and index are kept on the way. One thread per cell.
-template <typename T, bool RANDOM>
+// and colour u16(sum c w / sum w). One thread per cell; the winner is the minimum key of the
+// cell's own run, its exact coordinates and index read back through the key's source position.
+// The neighbourhood is walked as 9 x-rows, each one contiguous run of payloads. With the point in
+// neighbour cell m = c + d (d in {-1, 0, 1}^3) and its offset o from m's centre,
+//   centre(c) - p = -d t - o,
+// evaluated in float, and w = 2^(|centre(c) - p|^2 * (-10 log2(e) / diag^2)), which is the same
+// function as the reference's expression. The float evaluation moves a weight by a few parts in
+// 10^6, so a channel can differ by one where the average lies that close to an integer, the
+// same tolerance the float sums' order dependence already needs (the reference adds with
+// atomics). Two records per step, both loads in flight before either is weighed.
+template <typename T>
 __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
                                                         uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
                                                         const uint32_t* __restrict__ bitmap,
@@ -558,8 +587,9 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
                                                         const uint32_t* __restrict__ cellKey,
                                                         const uint32_t* __restrict__ cellNode,
                                                         const uint32_t* __restrict__ cellStart,
-                                                        const Payload<T, RANDOM>* __restrict__ payload,
-                                                        PointStore<T> ps, uint32_t* __restrict__ stageNode)
+                                                        const WeightPayload* __restrict__ payload,
+                                                        const uint64_t* __restrict__ winKey, PointStore<T> ps,
+                                                        uint32_t* __restrict__ stageNode)
 {
     // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
     const uint32_t b     = blockIdx.x;
@@ -608,37 +638,52 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
         lo[r] = count[r] ? cellStart[first[r]] : 0;
         hi[r] = count[r] ? cellStart[first[r] + count[r]] : 0;
     }
-    const uint32_t cLo = cellStart[c], cHi = cellStart[c + 1];
+
+    // the winner: minimum key of the cell's own run
+    uint64_t best = ~0ull;
+    for (uint32_t q = cellStart[c], e = cellStart[c + 1]; q < e; ++q)
+    {
+        const uint64_t k = winKey[q];
+        best             = k < best ? k : best;
+    }
+    const uint32_t wpos = static_cast<uint32_t>(best);
+    T bx, by, bz;
+    ps.xyz(wpos, bx, by, bz);
+    const uint32_t bestIdx = ps.index(wpos);
+
     const double t     = ng.grid.bbSize / gs;
-    const double ctrX  = ng.grid.minX + cx * t + (t * 0.5);
-    const double ctrY  = ng.grid.minY + cy * t + (t * 0.5);
-    const double ctrZ  = ng.grid.minZ + cz * t + (t * 0.5);
+    const float tf     = static_cast<float>(t);
+    const float kexp   = static_cast<float>((-10.0 * 1.4426950408889634) / (ng.diag * ng.diag));
+    const uint32_t cxm = (static_cast<uint32_t>(cx) + 3u) & 3u;  // (c.x - 1) mod 4: d.x = ((m.x - cxm) & 3) - 1
     float acc[4]       = {0.f, 0.f, 0.f, 0.f};
-    uint32_t bestRank  = ~0u, bestIdx = 0;
-    T bx = 0, by = 0, bz = 0;
     for (int r = 0; r < 9; ++r)
     {
-        for (uint32_t q = lo[r]; q < hi[r]; ++q)
+        const float uyc = -static_cast<float>((r / 3) - 1) * tf;  // -d.y t
+        const float uzc = -static_cast<float>((r % 3) - 1) * tf;  // -d.z t
+        auto weigh      = [&](const WeightPayload& pl) {
+            const int32_t dx = static_cast<int32_t>(((pl.rgbx >> 24) + 4u - cxm) & 3u) - 1;
+            const float ux   = __builtin_fmaf(-static_cast<float>(dx), tf, -pl.ox);
+            const float uy   = uyc - pl.oy;
+            const float uz   = uzc - pl.oz;
+            float d2         = ux * ux;
+            d2               = __builtin_fmaf(uy, uy, d2);
+            d2               = __builtin_fmaf(uz, uz, d2);
+            const float w    = __builtin_amdgcn_exp2f(d2 * kexp);
+            acc[0]           = __builtin_fmaf(static_cast<float>(pl.rgbx & 0xFF), w, acc[0]);
+            acc[1]           = __builtin_fmaf(static_cast<float>((pl.rgbx >> 8) & 0xFF), w, acc[1]);
+            acc[2]           = __builtin_fmaf(static_cast<float>((pl.rgbx >> 16) & 0xFF), w, acc[2]);
+            acc[3] += w;
+        };
+        uint32_t q = lo[r];
+        for (; q + 1 < hi[r]; q += 2)
         {
-            const Payload<T, RANDOM> pl = payload[q];
-            if (q >= cLo && q < cHi && pl.rank() < bestRank)
-            {
-                bestRank = pl.rank();
-                bestIdx  = pl.idx;
-                bx       = pl.x;
-                by       = pl.y;
-                bz       = pl.z;
-            }
-            const double uX = ctrX - static_cast<double>(pl.x);
-            const double uY = ctrY - static_cast<double>(pl.y);
-            const double uZ = ctrZ - static_cast<double>(pl.z);
-            float res       = static_cast<float>(sqrt(uX * uX + uY * uY + uZ * uZ) / ng.diag);
-            res             = expf(-((res * res) * 10.f));
-            acc[0] += static_cast<float>(pl.rgb & 0xFF) * res;
-            acc[1] += static_cast<float>((pl.rgb >> 8) & 0xFF) * res;
-            acc[2] += static_cast<float>((pl.rgb >> 16) & 0xFF) * res;
-            acc[3] += res;
+            const WeightPayload p0 = payload[q];
+            const WeightPayload p1 = payload[q + 1];
+            weigh(p0);
+            weigh(p1);
         }
+        if (q < hi[r])
+            weigh(payload[q]);
     }
     OutRecord rec = quantizePoint(bx, by, bz, 0, ps.cubic, ps.sx, ps.sy, ps.sz);
     rec.r         = static_cast<uint16_t>(acc[0] / acc[3]);
