@@ -29,6 +29,8 @@
 
 namespace oc {
 
+struct SubsampleParams;
+
 class DeviceBuffer
 {
 public:
@@ -108,6 +110,8 @@ private:
     template <typename T>
     void distributePoints();
     template <typename T>
+    bool orderLeafPoints(double cubicSize, const SubsampleParams& sp);
+    template <typename T>
     void subsampleLevels();
     void placeInternalNodes();
 
@@ -159,7 +163,8 @@ private:
 
     // per-level subsampling workspace: internal node list, input segments and tiles, per-node
     // occupancy bitmaps with their popcount prefix, per-cell (dense index) state, per-input state
-    DeviceBuffer itsInternalList, itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid, itsTileList;
+    DeviceBuffer itsLeafList, itsInternalList, itsSegCount, itsSegSrc, itsSegTile, itsSegInput, itsNodeGrid,
+            itsTileList;
     DeviceBuffer itsBitmap, itsPrefix, itsNodeFirst;
     DeviceBuffer itsWin, itsColor, itsCellCnt, itsCellStart, itsCellKey, itsCellNode;
     DeviceBuffer itsCellOf, itsCompactOf, itsRankOf, itsPayload;

@@ -217,7 +217,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
             &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
             &itsCellNode,    &itsCellOf,      &itsCompactOf,     &itsRankOf,       &itsPayload,     &itsScalars,
-            &itsScanScratch};
+            &itsScanScratch, &itsLeafList};
 }
 
 uint64_t OctreeBuilder::arenaBytes() const
@@ -252,7 +252,7 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     const uint64_t mBound = std::min<uint64_t>(dense, (depth + 1) * n);
     uint64_t bytes        = hostCloud ? n * cloud.dataStride : 0;
     bytes += 8 * dense;                                     // count pyramid, dense -> sparse
-    bytes += 8 * mBound + 69 * mBound;                      // node keys / counts, node arrays
+    bytes += 8 * mBound + 77 * mBound + n / 1024;           // node keys / counts, node arrays, leaf chunks
     bytes += 22 * e;                                        // lut + output records
     bytes += (2 * sortedPointBytes(cloud) + 2 + 4) * n;     // bucketed, sorted, fine codes, permutation
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
@@ -550,6 +550,48 @@ static void launchAccumulate(bool random, unsigned blocks, hipStream_t s, uint32
 }
 
 
+// Every leaf's range of the sorted cloud, reordered by bitmap word of its parent's grid into the
+// bucket buffer (free after the count stage), which then serves as the leaf point store.
+template <typename T>
+bool OctreeBuilder::orderLeafPoints(double cubicSize, const SubsampleParams& sp)
+{
+    uint64_t* sc = itsScalars.as<uint64_t>();
+    // chunks per leaf are at most N / LEAF_CHUNK + (number of leaves) in total
+    itsLeafList.ensure(sizeof(uint64_t) * (itsM + itsCloud.pointAmount / LEAF_CHUNK + 1));
+    itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(itsM) + 1));
+    scanExclusive<uint32_t>(itsM, LeafIn{itsNodeFlags.as<uint8_t>(), itsNodeParent.as<int32_t>(),
+                                         itsNodePoints.as<uint32_t>()},
+                            LeafOut{itsLeafList.as<uint64_t>()}, itsScanScratch.as<uint32_t>(),
+                            reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL), itsStream);
+    OC_LAUNCH_CHECK();
+    const uint32_t chunks = static_cast<uint32_t>(readDevice(sc + SC_TABLE_TOTAL));
+    if (chunks == 0)
+        return false;  // the root is the only leaf: nothing is subsampled
+    LeafOrderParams p{};
+    for (int a = 0; a < 3; ++a)
+        p.cubicMin[a] = sp.cubicMin[a];
+    for (uint32_t l = 0; l <= itsDepth; ++l)
+    {
+        p.sideHalf[l]  = sp.sideHalf[l];
+        p.side[l]      = cubicSize / std::pow(2.0, static_cast<double>(itsDepth - l));  // rp_subsampling.cu:17-18
+        p.lp.offset[l] = itsLevelOffset[l];
+        p.lp.grid[l]   = itsLevelGrid[l];
+    }
+    p.lp.depth     = itsDepth;
+    p.lp.threshold = itsProc.mergingThreshold;
+    p.g            = itsProc.subsamplingGrid;
+    const uint32_t h    = p.g / 2;
+    const uint64_t keys = rowsPerNode(h) * ((h + 31) / 32);
+    while (((keys - 1) >> p.keyShift) >= LEAF_KEYS)
+        ++p.keyShift;
+    kLeafCellOrder<T><<<chunks, LEAF_THREADS, 0, itsStream>>>(
+            itsLeafList.as<uint64_t>(), p, itsNodeKey.as<uint32_t>(), itsNodeParent.as<int32_t>(),
+            itsNodePoints.as<uint32_t>(), itsNodeSorted.as<uint32_t>(), itsSorted.as<SortedPoint<T>>(),
+            itsBucketed.as<SortedPoint<T>>());
+    OC_LAUNCH_CHECK();
+    return true;
+}
+
 template <typename T>
 void OctreeBuilder::subsampleLevels()
 {
@@ -584,6 +626,9 @@ void OctreeBuilder::subsampleLevels()
     sp.depth = itsDepth;
     sp.g     = g;
     sp.seed  = itsProc.seed;
+
+    if (g >= 2 && itsM > 1 && orderLeafPoints<T>(cubicSize, sp))
+        ps.sorted = itsBucketed.as<SortedPoint<T>>();  // the reordered copy
 
     for (uint32_t l = 1; l <= itsDepth; ++l)
     {

@@ -135,6 +135,29 @@ struct InternalOut
     }
 };
 
+// Subsampling grid of the node with linear index lin at level `level` (grid gl per axis): the
+// node minimum accumulated top-down exactly like the reference's DFS (calculateChildMinBB,
+// tools.cu:29-61; rp_subsampling.cu:16-60): x = min.x; for every step down whose slot has the x
+// bit, x = x + side/2 of that step; the extent is bbSize = (min.x + side) - min.x.
+__device__ __forceinline__ Gridding nodeGridding(uint32_t lin, uint32_t gl, uint32_t level, uint32_t depth,
+                                                 const double* cubicMin, const double* sideHalf, double side,
+                                                 uint32_t g)
+{
+    const uint32_t x = lin % gl, y = (lin / gl) % gl, z = lin / (gl * gl);
+    double mx = cubicMin[0], my = cubicMin[1], mz = cubicMin[2];
+    for (uint32_t L = depth; L > level; --L)
+    {
+        const uint32_t b = L - 1 - level;
+        if ((x >> b) & 1)
+            mx = mx + sideHalf[L];
+        if ((y >> b) & 1)
+            my = my + sideHalf[L];
+        if ((z >> b) & 1)
+            mz = mz + sideHalf[L];
+    }
+    return Gridding{mx, my, mz, (mx + side) - mx, g};
+}
+
 // Per internal node k of the level: its eight input segments (children in slot order, the leaf's
 // range of the sorted cloud or the child's staged subsample) and its gridding. The node minimum is accumulated top-down
 // exactly like the reference's DFS (calculateChildMinBB, tools.cu:29-61; rp_subsampling.cu:16-60):
@@ -166,33 +189,189 @@ __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsamplePara
             segCount[8 * k + s] = cnt;
             segSrc[8 * k + s]   = src;
         }
-        uint32_t key = nodeKey[id];
-        uint32_t lin = key - sp.levelOffset;
-        uint32_t gl  = sp.levelGrid;
-        uint32_t x = lin % gl, y = (lin / gl) % gl, z = lin / (gl * gl);
-        double mx = sp.cubicMin[0], my = sp.cubicMin[1], mz = sp.cubicMin[2];
-        for (uint32_t L = sp.depth; L > sp.level; --L)
-        {
-            uint32_t b = L - 1 - sp.level;
-            if ((x >> b) & 1)
-                mx = mx + sp.sideHalf[L];
-            if ((y >> b) & 1)
-                my = my + sp.sideHalf[L];
-            if ((z >> b) & 1)
-                mz = mz + sp.sideHalf[L];
-        }
+        const uint32_t key = nodeKey[id];
         NodeGrid ng;
-        ng.grid.minX     = mx;
-        ng.grid.minY     = my;
-        ng.grid.minZ     = mz;
-        ng.grid.bbSize   = (mx + sp.side) - mx;
-        ng.grid.gridSize = sp.g;
+        ng.grid          = nodeGridding(key - sp.levelOffset, sp.levelGrid, sp.level, sp.depth, sp.cubicMin,
+                                        sp.sideHalf, sp.side, sp.g);
         double t         = ng.grid.bbSize / sp.g;
         ng.diag          = sqrt((t * 3) * (t * 3) * 3) / 2.0;
         ng.salt          = nodeSalt(sp.seed, key);
         ng.id            = id;
         nodeGrid[k]      = ng;
     }
+}
+
+// Leaf point order for subsampling. Within a leaf the points keep the chunking sort's order (its
+// finest cells), which says nothing about the parent's subsampling grid: a leaf spans 64^3 of the
+// parent's 128^3 cells, so consecutive points land in unrelated bitmap words and cells, and every
+// atomic, rank lookup and payload store of the level is random. Every leaf range of the sorted
+// cloud is cut into chunks of at most LEAF_CHUNK points, and one workgroup reorders a chunk by
+// bitmap word in the parent grid (the cell rows of the leaf's octant in the bitmap's pencil order,
+// 32 cells per word) with a counting sort over an LDS histogram; chunks bound the work of one
+// workgroup (a dense cluster puts 10^5 points in one finest cell). Points within a leaf are
+// unordered by definition (node point sets are compared), so the order is free to choose; the key
+// only orders, so a point the parent grid clamps lands in an edge key.
+constexpr int LEAF_THREADS    = 1024;
+constexpr int LEAF_ITEMS      = 4;                          // points per thread, held in registers
+constexpr uint32_t LEAF_KEYS  = 8192;                       // word keys (histogram entries)
+constexpr uint32_t LEAF_CHUNK = LEAF_THREADS * LEAF_ITEMS;  // points per workgroup
+
+struct LeafOrderParams
+{
+    double cubicMin[3];
+    double sideHalf[MAX_LEVELS];
+    double side[MAX_LEVELS];  // node side per level
+    LevelParams lp;
+    uint32_t g;         // subsampling grid (>= 2)
+    uint32_t keyShift;  // word keys >> keyShift < LEAF_KEYS
+};
+
+// chunks of a leaf that a subsampling level reads (finished, not internal, with a parent)
+struct LeafIn
+{
+    const uint8_t* flags;
+    const int32_t* parent;
+    const uint32_t* count;
+    __device__ uint32_t operator()(uint64_t i) const
+    {
+        return ((flags[i] & (NODE_FINISHED | NODE_INTERNAL)) == NODE_FINISHED && parent[i] >= 0)
+                       ? (count[i] + LEAF_CHUNK - 1) / LEAF_CHUNK
+                       : 0u;
+    }
+};
+
+// work list entry per chunk: chunk number << 32 | leaf id
+struct LeafOut
+{
+    uint64_t* list;
+    __device__ void operator()(uint64_t i, uint32_t prefix, uint32_t v) const
+    {
+        for (uint32_t c = 0; c < v; ++c)
+            list[prefix + c] = (static_cast<uint64_t>(c) << 32) | i;
+    }
+};
+
+// histogram slot of a key: one pad word per 32 keys, so the per-thread runs of the scan fall in
+// distinct banks
+__device__ __forceinline__ uint32_t leafSlot(uint32_t key)
+{
+    return key + (key >> 5);
+}
+
+__device__ __forceinline__ uint32_t leafLocal(uint64_t i, uint32_t origin, uint32_t h)
+{
+    return i < origin ? 0u : static_cast<uint32_t>(i - origin < h ? i - origin : h - 1);
+}
+
+// Word key of a point: its cell in the parent grid (multiplied by 1 / t instead of divided by t:
+// the key only orders, so a point next to a cell face may take either side) relative to the leaf's
+// octant, as (pencil row, x / 32) in the bitmap's order.
+struct LeafKeyer
+{
+    Gridding grid;
+    double inv;  // gridSize / bbSize
+    uint32_t ox, oy, oz, h, wordsPerRow, shift;
+    __device__ __forceinline__ uint32_t operator()(double x, double y, double z) const
+    {
+        const double top = grid.gridSize - 1.0;
+        const uint64_t ix = static_cast<uint64_t>(static_cast<int64_t>(fmin((x - grid.minX) * inv, top)));
+        const uint64_t iy = static_cast<uint64_t>(static_cast<int64_t>(fmin((y - grid.minY) * inv, top)));
+        const uint64_t iz = static_cast<uint64_t>(static_cast<int64_t>(fmin((z - grid.minZ) * inv, top)));
+        const uint32_t lx = leafLocal(ix, ox, h), ly = leafLocal(iy, oy, h), lz = leafLocal(iz, oz, h);
+        return (rowIndex(ly, lz, h) * wordsPerRow + (lx >> 5)) >> shift;
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* __restrict__ chunkList, LeafOrderParams p,
+                                                                const uint32_t* __restrict__ nodeKey,
+                                                                const int32_t* __restrict__ parent,
+                                                                const uint32_t* __restrict__ count,
+                                                                const uint32_t* __restrict__ sortedStart,
+                                                                const SortedPoint<T>* __restrict__ in,
+                                                                SortedPoint<T>* __restrict__ out)
+{
+    __shared__ uint32_t hist[LEAF_KEYS + LEAF_KEYS / 32];
+    __shared__ uint16_t perm[LEAF_CHUNK];
+    __shared__ uint32_t scanLds[LEAF_THREADS / 64];
+    const uint64_t entry = chunkList[blockIdx.x];
+    const uint32_t leaf  = static_cast<uint32_t>(entry);
+    const uint32_t first = static_cast<uint32_t>(entry >> 32) * LEAF_CHUNK;
+    const uint32_t total = count[leaf];
+    const uint32_t src   = sortedStart[leaf] + first;
+    const uint32_t cnt   = total - first < LEAF_CHUNK ? total - first : LEAF_CHUNK;
+    const uint32_t pkey  = nodeKey[static_cast<uint32_t>(parent[leaf])];
+    const uint32_t pl    = levelOfKey(pkey, p.lp);
+    LeafKeyer keyer;
+    keyer.grid = nodeGridding(pkey - p.lp.offset[pl], p.lp.grid[pl], pl, p.lp.depth, p.cubicMin, p.sideHalf,
+                              p.side[pl], p.g);
+    keyer.inv  = keyer.grid.gridSize / keyer.grid.bbSize;
+    // the leaf's octant of the parent grid: the low bit of its coordinates one level down
+    const uint32_t ll = pl - 1, lgl = p.lp.grid[ll], llin = nodeKey[leaf] - p.lp.offset[ll];
+    keyer.h           = p.g / 2;
+    keyer.ox          = ((llin % lgl) & 1u) * keyer.h;
+    keyer.oy          = (((llin / lgl) % lgl) & 1u) * keyer.h;
+    keyer.oz          = ((llin / (lgl * lgl)) & 1u) * keyer.h;
+    keyer.wordsPerRow = (keyer.h + 31) / 32;
+    keyer.shift       = p.keyShift;
+
+    for (uint32_t i = threadIdx.x; i < LEAF_KEYS + LEAF_KEYS / 32; i += LEAF_THREADS)
+        hist[i] = 0;
+    __syncthreads();
+    // all loads in flight first, then the keys and their ranks within the key
+    uint32_t key[LEAF_ITEMS], rank[LEAF_ITEMS];
+    {
+        T x[LEAF_ITEMS], y[LEAF_ITEMS], z[LEAF_ITEMS];
+#pragma unroll
+        for (int k = 0; k < LEAF_ITEMS; ++k)
+        {
+            const uint32_t i = threadIdx.x + k * LEAF_THREADS;
+            if (i < cnt)
+            {
+                const SortedPoint<T>& q = in[src + i];
+                x[k]                    = q.x;
+                y[k]                    = q.y;
+                z[k]                    = q.z;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < LEAF_ITEMS; ++k)
+            if (threadIdx.x + k * LEAF_THREADS < cnt)
+            {
+                key[k]  = keyer(static_cast<double>(x[k]), static_cast<double>(y[k]), static_cast<double>(z[k]));
+                rank[k] = atomicAdd(hist + leafSlot(key[k]), 1u);
+            }
+    }
+    __syncthreads();
+
+    // exclusive scan of the histogram, LEAF_KEYS / LEAF_THREADS consecutive keys per thread
+    constexpr uint32_t PER = LEAF_KEYS / LEAF_THREADS;
+    const uint32_t base    = threadIdx.x * PER;
+    uint32_t sum           = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k)
+        sum += hist[leafSlot(base + k)];
+    uint32_t all;
+    uint32_t off = sortBlockExclusiveScanN<LEAF_THREADS>(sum, all, scanLds);
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k)
+    {
+        const uint32_t v         = hist[leafSlot(base + k)];
+        hist[leafSlot(base + k)] = off;
+        off += v;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < LEAF_ITEMS; ++k)
+    {
+        const uint32_t i = threadIdx.x + k * LEAF_THREADS;
+        if (i < cnt)
+            perm[hist[leafSlot(key[k])] + rank[k]] = static_cast<uint16_t>(i);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < cnt; j += LEAF_THREADS)
+        out[src + j] = in[src + perm[j]];
 }
 
 // packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums
