@@ -121,6 +121,10 @@ private:
 
     int itsDevice;
     hipStream_t itsStream = nullptr;
+    // Size readbacks: a one-thread kernel publishes {value, sequence} into coherent pinned host
+    // memory and the host spins on the sequence (no staging copy, no blocking-sync wake-up).
+    uint64_t* itsMailbox  = nullptr;
+    uint64_t itsMailSeq   = 0;
     bool itsReady         = false;  // setup() has run
     bool itsDetailedTimings = true;
     CloudInfo itsCloud;

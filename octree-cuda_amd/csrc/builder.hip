@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -120,6 +121,9 @@ OctreeBuilder::OctreeBuilder(int device) : itsDevice(device)
     OC_HIP_CHECK(hipStreamCreateWithFlags(&itsStream, hipStreamNonBlocking));
     for (auto& ev : itsEvents)
         OC_HIP_CHECK(hipEventCreate(&ev));
+    OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&itsMailbox), 2 * sizeof(uint64_t), hipHostMallocCoherent));
+    itsMailbox[0] = 0;
+    itsMailbox[1] = 0;
 }
 
 OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc,
@@ -279,6 +283,8 @@ OctreeBuilder::~OctreeBuilder()
         for (auto& ev : quad)
             if (ev)
                 (void)hipEventDestroy(ev);
+    if (itsMailbox)
+        (void)hipHostFree(itsMailbox);
 }
 
 std::array<hipEvent_t, 4>& OctreeBuilder::batchEvents()
@@ -298,13 +304,40 @@ void OctreeBuilder::syncStream()
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 
+// Publishes a device value to the host mailbox: the value first, then (after a system-scope
+// release) the sequence number the host waits for.
+template <typename V>
+__global__ void kPublish(const V* __restrict__ src, uint64_t* box, uint64_t seq)
+{
+    if (threadIdx.x != 0)
+        return;
+    __hip_atomic_store(box, static_cast<uint64_t>(*src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(box + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// A size the host needs to size the next launches. The stream is drained up to the publish: the
+// host spins on the mailbox for up to 2 ms (the common case: the value is ready within microseconds
+// of the last kernel) and then falls back to a blocking synchronisation, which also reports a
+// failed kernel instead of waiting forever.
 template <typename V>
 V OctreeBuilder::readDevice(const V* devicePtr)
 {
-    V v{};
-    OC_HIP_CHECK(hipMemcpyAsync(&v, devicePtr, sizeof(V), hipMemcpyDeviceToHost, itsStream));
-    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
-    return v;
+    const uint64_t seq = ++itsMailSeq;
+    kPublish<V><<<1, 64, 0, itsStream>>>(devicePtr, itsMailbox, seq);
+    OC_LAUNCH_CHECK();
+    const auto start = std::chrono::steady_clock::now();
+    while (__atomic_load_n(itsMailbox + 1, __ATOMIC_ACQUIRE) != seq)
+    {
+        __builtin_ia32_pause();
+        if (std::chrono::steady_clock::now() - start > std::chrono::milliseconds(2))
+        {
+            OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+            if (__atomic_load_n(itsMailbox + 1, __ATOMIC_ACQUIRE) != seq)
+                throw std::runtime_error("device value was not published");
+            break;
+        }
+    }
+    return static_cast<V>(__atomic_load_n(itsMailbox, __ATOMIC_ACQUIRE));
 }
 
 void OctreeBuilder::generate()
