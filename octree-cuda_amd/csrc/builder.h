@@ -29,7 +29,6 @@
 
 namespace oc {
 
-struct SubsampleParams;
 
 class DeviceBuffer
 {
@@ -110,7 +109,7 @@ private:
     template <typename T>
     void distributePoints();
     template <typename T>
-    bool orderLeafPoints(double cubicSize, const SubsampleParams& sp);
+    bool orderLeafPoints();
     template <typename T>
     void subsampleLevels();
     void placeInternalNodes();
@@ -185,6 +184,8 @@ private:
         EV_COUNT_END,
         EV_MERGE_END,
         EV_DIST_BEGIN,
+        EV_ORDER_BEGIN,
+        EV_ORDER_END,
         EV_DIST_END,
         EV_END,
         EV_SLOTS

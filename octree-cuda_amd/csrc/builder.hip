@@ -383,9 +383,10 @@ void OctreeBuilder::generateTyped()
         return v;
     };
     itsTimings.assign(7, 0.f);
-    itsTimings[0] = ms(EV_COUNT_BEGIN, EV_COUNT_END);
-    itsTimings[1] = ms(EV_COUNT_END, EV_MERGE_END);
-    itsTimings[2] = ms(EV_DIST_BEGIN, EV_DIST_END);
+    const float order = ms(EV_ORDER_BEGIN, EV_ORDER_END);  // moving the records: count stage
+    itsTimings[0]     = ms(EV_COUNT_BEGIN, EV_COUNT_END) + order;
+    itsTimings[1]     = ms(EV_COUNT_END, EV_MERGE_END);
+    itsTimings[2]     = ms(EV_DIST_BEGIN, EV_DIST_END) - order;
     itsTimings[3] = ms(EV_DIST_END, EV_END);
     itsTimings[4] = ms(EV_START, EV_END);
     for (uint32_t i = 0; i < itsBatchesRun; ++i)
@@ -439,11 +440,7 @@ void OctreeBuilder::countPoints()
     kFineSort<<<nb, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart, fp,
                                                   itsCount.as<uint32_t>(), itsPerm.as<uint32_t>());
     OC_LAUNCH_CHECK();
-    kGatherSorted<T><<<gatherGrid(n), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
-                                                                 itsPerm.as<uint32_t>(), n,
-                                                                 itsSorted.as<SortedPoint<T>>());
-    OC_LAUNCH_CHECK();
-    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));
+    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));  // the records move once the leaves are known
 }
 
 void OctreeBuilder::mergeLevels()
@@ -553,6 +550,18 @@ void OctreeBuilder::distributePoints()
                                                          itsNodeParent.as<int32_t>(), itsNodeChild.as<int32_t>(),
                                                          itsNodeSorted.as<uint32_t>());
     OC_LAUNCH_CHECK();
+    // the sorted cloud (part of the count stage's work): leaf and cell order when there are
+    // internal nodes, plain Morton order otherwise
+    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_ORDER_BEGIN], itsStream));
+    if (!orderLeafPoints<T>())
+    {
+        const uint32_t n = itsCloud.pointAmount;
+        kGatherSorted<T><<<gatherGrid(n), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
+                                                                     itsPerm.as<uint32_t>(), n,
+                                                                     itsSorted.as<SortedPoint<T>>());
+        OC_LAUNCH_CHECK();
+    }
+    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_ORDER_END], itsStream));
     kDistributeSorted<T><<<gridFor(itsCloud.pointAmount, 16384), BLOCK, 0, itsStream>>>(
             itsSorted.as<SortedPoint<T>>(), itsCloud.pointAmount, g, itsCount.as<uint32_t>(),
             itsNodeSorted.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),
@@ -583,30 +592,33 @@ static void launchAccumulate(bool random, unsigned blocks, hipStream_t s, uint32
 }
 
 
-// Every leaf's range of the sorted cloud, reordered by bitmap word of its parent's grid into the
-// bucket buffer (free after the count stage), which then serves as the leaf point store.
+// The sorted cloud in leaf and cell order (kLeafCellOrder): false when no leaf has a parent (the
+// root is the only leaf), so nothing is subsampled and plain Morton order serves.
 template <typename T>
-bool OctreeBuilder::orderLeafPoints(double cubicSize, const SubsampleParams& sp)
+bool OctreeBuilder::orderLeafPoints()
 {
+    if (itsProc.subsamplingGrid < 2 || itsM < 2)
+        return false;
     uint64_t* sc = itsScalars.as<uint64_t>();
     // chunks per leaf are at most N / LEAF_CHUNK + (number of leaves) in total
     itsLeafList.ensure(sizeof(uint64_t) * (itsM + itsCloud.pointAmount / LEAF_CHUNK + 1));
     itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(itsM) + 1));
     scanExclusive<uint32_t>(itsM, LeafIn{itsNodeFlags.as<uint8_t>(), itsNodeParent.as<int32_t>(),
-                                         itsNodePoints.as<uint32_t>()},
+                                         itsNodeCount.as<uint32_t>()},
                             LeafOut{itsLeafList.as<uint64_t>()}, itsScanScratch.as<uint32_t>(),
                             reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL), itsStream);
     OC_LAUNCH_CHECK();
     const uint32_t chunks = static_cast<uint32_t>(readDevice(sc + SC_TABLE_TOTAL));
     if (chunks == 0)
         return false;  // the root is the only leaf: nothing is subsampled
+    const double cubicSize = itsCloud.bbMax[0] - itsCloud.bbMin[0];  // metadata.cuh:66-69
     LeafOrderParams p{};
     for (int a = 0; a < 3; ++a)
-        p.cubicMin[a] = sp.cubicMin[a];
+        p.cubicMin[a] = itsCloud.bbMin[a];
     for (uint32_t l = 0; l <= itsDepth; ++l)
     {
-        p.sideHalf[l]  = sp.sideHalf[l];
         p.side[l]      = cubicSize / std::pow(2.0, static_cast<double>(itsDepth - l));  // rp_subsampling.cu:17-18
+        p.sideHalf[l]  = p.side[l] / 2.0;
         p.lp.offset[l] = itsLevelOffset[l];
         p.lp.grid[l]   = itsLevelGrid[l];
     }
@@ -619,8 +631,8 @@ bool OctreeBuilder::orderLeafPoints(double cubicSize, const SubsampleParams& sp)
         ++p.keyShift;
     kLeafCellOrder<T><<<chunks, LEAF_THREADS, 0, itsStream>>>(
             itsLeafList.as<uint64_t>(), p, itsNodeKey.as<uint32_t>(), itsNodeParent.as<int32_t>(),
-            itsNodePoints.as<uint32_t>(), itsNodeSorted.as<uint32_t>(), itsSorted.as<SortedPoint<T>>(),
-            itsBucketed.as<SortedPoint<T>>());
+            itsNodeCount.as<uint32_t>(), itsNodeSorted.as<uint32_t>(), itsBucketed.as<SortedPoint<T>>(),
+            itsPerm.as<uint32_t>(), itsSorted.as<SortedPoint<T>>());
     OC_LAUNCH_CHECK();
     return true;
 }
@@ -659,9 +671,6 @@ void OctreeBuilder::subsampleLevels()
     sp.depth = itsDepth;
     sp.g     = g;
     sp.seed  = itsProc.seed;
-
-    if (g >= 2 && itsM > 1 && orderLeafPoints<T>(cubicSize, sp))
-        ps.sorted = itsBucketed.as<SortedPoint<T>>();  // the reordered copy
 
     for (uint32_t l = 1; l <= itsDepth; ++l)
     {

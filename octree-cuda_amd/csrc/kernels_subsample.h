@@ -201,16 +201,18 @@ __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsamplePara
     }
 }
 
-// Leaf point order for subsampling. Within a leaf the points keep the chunking sort's order (its
-// finest cells), which says nothing about the parent's subsampling grid: a leaf spans 64^3 of the
-// parent's 128^3 cells, so consecutive points land in unrelated bitmap words and cells, and every
-// atomic, rank lookup and payload store of the level is random. Every leaf range of the sorted
-// cloud is cut into chunks of at most LEAF_CHUNK points, and one workgroup reorders a chunk by
-// bitmap word in the parent grid (the cell rows of the leaf's octant in the bitmap's pencil order,
-// 32 cells per word) with a counting sort over an LDS histogram; chunks bound the work of one
-// workgroup (a dense cluster puts 10^5 points in one finest cell). Points within a leaf are
-// unordered by definition (node point sets are compared), so the order is free to choose; the key
-// only orders, so a point the parent grid clamps lands in an edge key.
+// The sorted cloud in leaf and cell order. The fine sort's permutation puts the points in Morton
+// order of their finest cells, so every leaf is one range; within a leaf that order says nothing
+// about the parent's subsampling grid (a leaf spans 64^3 of the parent's 128^3 cells), and
+// subsampling would see consecutive points in unrelated bitmap words and cells, making every
+// atomic, rank lookup and payload store of the level random. So the records are gathered through
+// the permutation leaf by leaf and reordered by bitmap word in the parent grid (the cell rows of
+// the leaf's octant in the bitmap's pencil order, 32 cells per word): every leaf range is cut into
+// chunks of at most LEAF_CHUNK points, one workgroup sorts a chunk with a counting sort over an
+// LDS histogram and writes it in order. Chunks bound the work of one workgroup (a dense cluster
+// puts 10^5 points in one finest cell). Points within a leaf are unordered by definition (node
+// point sets are compared), so the order is free to choose; the key only orders, so a point the
+// parent grid clamps lands in an edge key.
 constexpr int LEAF_THREADS    = 1024;
 constexpr int LEAF_ITEMS      = 4;                          // points per thread, held in registers
 constexpr uint32_t LEAF_KEYS  = 8192;                       // word keys (histogram entries)
@@ -288,11 +290,12 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
                                                                 const int32_t* __restrict__ parent,
                                                                 const uint32_t* __restrict__ count,
                                                                 const uint32_t* __restrict__ sortedStart,
-                                                                const SortedPoint<T>* __restrict__ in,
+                                                                const SortedPoint<T>* __restrict__ bucketed,
+                                                                const uint32_t* __restrict__ perm,
                                                                 SortedPoint<T>* __restrict__ out)
 {
     __shared__ uint32_t hist[LEAF_KEYS + LEAF_KEYS / 32];
-    __shared__ uint16_t perm[LEAF_CHUNK];
+    __shared__ uint16_t order[LEAF_CHUNK];
     __shared__ uint32_t scanLds[LEAF_THREADS / 64];
     const uint64_t entry = chunkList[blockIdx.x];
     const uint32_t leaf  = static_cast<uint32_t>(entry);
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
         hist[i] = 0;
     __syncthreads();
     // all loads in flight first, then the keys and their ranks within the key
-    uint32_t key[LEAF_ITEMS], rank[LEAF_ITEMS];
+    uint32_t key[LEAF_ITEMS], rank[LEAF_ITEMS], from[LEAF_ITEMS];
     {
         T x[LEAF_ITEMS], y[LEAF_ITEMS], z[LEAF_ITEMS];
 #pragma unroll
@@ -327,8 +330,15 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
         {
             const uint32_t i = threadIdx.x + k * LEAF_THREADS;
             if (i < cnt)
+                from[k] = perm[src + i];
+        }
+#pragma unroll
+        for (int k = 0; k < LEAF_ITEMS; ++k)
+        {
+            const uint32_t i = threadIdx.x + k * LEAF_THREADS;
+            if (i < cnt)
             {
-                const SortedPoint<T>& q = in[src + i];
+                const SortedPoint<T>& q = bucketed[from[k]];
                 x[k]                    = q.x;
                 y[k]                    = q.y;
                 z[k]                    = q.z;
@@ -367,11 +377,11 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
     {
         const uint32_t i = threadIdx.x + k * LEAF_THREADS;
         if (i < cnt)
-            perm[hist[leafSlot(key[k])] + rank[k]] = static_cast<uint16_t>(i);
+            order[hist[leafSlot(key[k])] + rank[k]] = static_cast<uint16_t>(i);
     }
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < cnt; j += LEAF_THREADS)
-        out[src + j] = in[src + perm[j]];
+        out[src + j] = bucketed[perm[src + order[j]]];
 }
 
 // packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums
