@@ -209,10 +209,10 @@ __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsamplePara
 // the permutation leaf by leaf and reordered by bitmap word in the parent grid (the cell rows of
 // the leaf's octant in the bitmap's pencil order, 32 cells per word): every leaf range is cut into
 // chunks of at most LEAF_CHUNK points, one workgroup sorts a chunk with a counting sort over an
-// LDS histogram and writes it in order. Chunks bound the work of one workgroup (a dense cluster
-// puts 10^5 points in one finest cell). Points within a leaf are unordered by definition (node
-// point sets are compared), so the order is free to choose; the key only orders, so a point the
-// parent grid clamps lands in an edge key.
+// LDS histogram and stores every record at its place in the chunk's range. Chunks bound the work
+// of one workgroup (a dense cluster puts 10^5 points in one finest cell). Points within a leaf
+// are unordered by definition (node point sets are compared), so the order is free to choose;
+// the key only orders, so a point the parent grid clamps lands in an edge key.
 constexpr int LEAF_THREADS    = 1024;
 constexpr int LEAF_ITEMS      = 4;                          // points per thread, held in registers
 constexpr uint32_t LEAF_KEYS  = 8192;                       // word keys (histogram entries)
@@ -295,7 +295,6 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
                                                                 SortedPoint<T>* __restrict__ out)
 {
     __shared__ uint32_t hist[LEAF_KEYS + LEAF_KEYS / 32];
-    __shared__ uint16_t order[LEAF_CHUNK];
     __shared__ uint32_t scanLds[LEAF_THREADS / 64];
     const uint64_t entry = chunkList[blockIdx.x];
     const uint32_t leaf  = static_cast<uint32_t>(entry);
@@ -322,9 +321,12 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
         hist[i] = 0;
     __syncthreads();
     // all loads in flight first, then the keys and their ranks within the key
-    uint32_t key[LEAF_ITEMS], rank[LEAF_ITEMS], from[LEAF_ITEMS];
+    // every record is gathered once, kept in registers and stored at its place in the chunk's
+    // range (a few hundred KB, so the scattered stores merge into whole lines in L2)
+    uint32_t key[LEAF_ITEMS], rank[LEAF_ITEMS];
+    SortedPoint<T> rec[LEAF_ITEMS];
     {
-        T x[LEAF_ITEMS], y[LEAF_ITEMS], z[LEAF_ITEMS];
+        uint32_t from[LEAF_ITEMS];
 #pragma unroll
         for (int k = 0; k < LEAF_ITEMS; ++k)
         {
@@ -334,21 +336,14 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
         }
 #pragma unroll
         for (int k = 0; k < LEAF_ITEMS; ++k)
-        {
-            const uint32_t i = threadIdx.x + k * LEAF_THREADS;
-            if (i < cnt)
-            {
-                const SortedPoint<T>& q = bucketed[from[k]];
-                x[k]                    = q.x;
-                y[k]                    = q.y;
-                z[k]                    = q.z;
-            }
-        }
+            if (threadIdx.x + k * LEAF_THREADS < cnt)
+                rec[k] = bucketed[from[k]];
 #pragma unroll
         for (int k = 0; k < LEAF_ITEMS; ++k)
             if (threadIdx.x + k * LEAF_THREADS < cnt)
             {
-                key[k]  = keyer(static_cast<double>(x[k]), static_cast<double>(y[k]), static_cast<double>(z[k]));
+                key[k]  = keyer(static_cast<double>(rec[k].x), static_cast<double>(rec[k].y),
+                                static_cast<double>(rec[k].z));
                 rank[k] = atomicAdd(hist + leafSlot(key[k]), 1u);
             }
     }
@@ -374,14 +369,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
 
 #pragma unroll
     for (int k = 0; k < LEAF_ITEMS; ++k)
-    {
-        const uint32_t i = threadIdx.x + k * LEAF_THREADS;
-        if (i < cnt)
-            order[hist[leafSlot(key[k])] + rank[k]] = static_cast<uint16_t>(i);
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < cnt; j += LEAF_THREADS)
-        out[src + j] = bucketed[perm[src + order[j]]];
+        if (threadIdx.x + k * LEAF_THREADS < cnt)
+            out[src + hist[leafSlot(key[k])] + rank[k]] = rec[k];
 }
 
 // packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums
