@@ -13,10 +13,12 @@
 //   kFineSort       one workgroup per bucket: LDS histogram over the bucket's (2^lc)^3 level-0
 //                   cells, exclusive scan, dense counts of levels 0..lc of the count pyramid,
 //                   then the permutation into full Morton order (4-byte indices)
-//   kGatherSorted   the records gathered through the permutation, written in order
-// In Morton order every octree node is one contiguous range of the sorted cloud, so after the
-// merge a leaf's points are the range [sortedStart, sortedStart + count) and the distribution is
-// a coalesced copy of those ranges into the leaf data blocks (kDistributeSorted).
+// In Morton order every octree node is one contiguous range, so after the merge a leaf's points
+// are the range [sortedStart, sortedStart + count). The records move once more, then: gathered
+// through the permutation and ordered within every leaf by its parent's subsampling cells
+// (kLeafCellOrder, with the subsampling kernels), or in plain Morton order (kGatherSorted) when
+// the root is the only leaf. The distribution is a coalesced copy of the leaf ranges of that
+// sorted cloud into the leaf data blocks (kDistributeSorted).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -10,8 +10,12 @@
 //   prefix scan  popcounts of all bitmap words -> dense index of every occupied cell
 //   kAccumulate  winner per cell (atomicMin of rank<<32 | input position), packed colour sums,
 //                per-cell point counts for the weighted filter
-//   kScatterPayload (weighted only) point coordinates + colour in cell order
+//   kScatterPayload (weighted only) each point's offset from its cell centre + colour, and its
+//                winner key, in cell order
 //   kEmit        one output record per occupied cell, in ascending cell order per node
+//                (kEmitWeighted for the distance-weighted filter)
+// Leaf points come from the sorted cloud, whose leaf ranges kLeafCellOrder has put in the cell
+// order of each leaf's parent grid, so a level's first reads of them follow the bitmap.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,8 +62,8 @@ __host__ __device__ __forceinline__ void rowCoords(uint32_t row, uint32_t g, uin
     z = (pencil / p) * PENCIL + inside / PENCIL;
 }
 
-// The points a subsampling level reads: leaf points at positions [0, N) of the Morton-sorted
-// cloud (a leaf's points are one contiguous range there) and already subsampled points of lower
+// The points a subsampling level reads: leaf points at positions [0, N) of the sorted cloud (a
+// leaf's points are one contiguous range there) and already subsampled points of lower
 // levels at positions [N, N + I) (staging, in level order until the final DFS post-order
 // placement). A leaf point's output record is derived from its sorted record exactly as the
 // distribution writes it.
@@ -159,10 +163,7 @@ __device__ __forceinline__ Gridding nodeGridding(uint32_t lin, uint32_t gl, uint
 }
 
 // Per internal node k of the level: its eight input segments (children in slot order, the leaf's
-// range of the sorted cloud or the child's staged subsample) and its gridding. The node minimum is accumulated top-down
-// exactly like the reference's DFS (calculateChildMinBB, tools.cu:29-61; rp_subsampling.cu:16-60):
-// x = min.x; for every step down whose slot has the x bit, x = x + side/2 of that step; the grid
-// extent is bbSize = (min.x + side) - min.x.
+// range of the sorted cloud or the child's staged subsample) and its gridding (nodeGridding).
 __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp,
                                                     const uint32_t* __restrict__ internalList,
                                                     const uint32_t* __restrict__ nodeKey,
