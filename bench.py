@@ -89,9 +89,11 @@ def cpu_baseline(cloud, wl, cubic, sample_points, threads):
     import dataclasses
     s = wl.cloud
     sample = cloud[: sample_points * s.stride]
-    # a prefix subsamples relatively more than the whole cloud, so give it room (the output
-    # capacity only sizes buffers; the work is the same)
-    spec = dataclasses.replace(wl.build, output_factor=max(wl.build.output_factor, 3.0))
+    spec = wl.build
+    if sample_points < s.points:
+        # a prefix subsamples relatively more than the whole cloud, so give it room (the output
+        # capacity only sizes buffers; the work is the same)
+        spec = dataclasses.replace(wl.build, output_factor=max(wl.build.output_factor, 3.0))
     t0 = time.perf_counter()
     r = O.build(sample, sample_points, s.stride, s.cloud_type, cubic, [s.scale] * 3, spec, threads=threads,
                 want_data=False)
@@ -99,9 +101,10 @@ def cpu_baseline(cloud, wl, cubic, sample_points, threads):
     if r.status != 0:
         log(f"bench: CPU oracle failed with status {r.status}")
         return None
+    what = "the whole cloud" if sample_points == s.points else f"the first {sample_points:,} points of the same cloud"
     return {"value": sample_points / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (std::thread x {threads}) building the first {sample_points:,} points of the "
-                      f"same cloud with the same bounding box and configuration once, {dt:.1f} s"}
+            "sample": f"CPU oracle (std::thread x {threads}) building {what} with the same bounding box and "
+                      f"configuration once, {dt:.1f} s"}
 
 
 def run_batched(args):
@@ -162,8 +165,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="C3", choices=sorted(W.WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=16_000_000,
-                    help="points of the cloud the CPU oracle baseline builds (-1 = all, 0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="points of the cloud the CPU oracle baseline builds with all cores (-1 = all, 0 = skip)")
+    ap.add_argument("--cpu-sample-1t", type=int, default=4_000_000,
+                    help="points of the cloud the single-thread CPU oracle baseline builds (0 = skip)")
     ap.add_argument("--batched", type=int, default=1, help="also run the batched-build workload (0 = skip)")
     ap.add_argument("--serving-rates", default="10,20,40,80",
                     help="Poisson arrival rates (req/s) of the serving sweep, '' to skip")
@@ -246,6 +251,12 @@ def main():
         sample = s.points if args.cpu_sample < 0 else min(args.cpu_sample, s.points)
         log(f"bench: CPU oracle baseline on {sample:,} points with {threads} threads")
         cpu = cpu_baseline(cloud, wl, cubic, sample, threads)
+        if cpu is not None and args.cpu_sample_1t > 0:
+            sample1 = min(args.cpu_sample_1t, s.points)
+            log(f"bench: CPU oracle baseline on {sample1:,} points with 1 thread")
+            one = cpu_baseline(cloud, wl, cubic, sample1, 1)
+            if one is not None:
+                cpu["single_thread"] = {"value": one["value"], "unit": one["unit"], "sample": one["sample"]}
 
     line = {
         "metric": "points/sec full octree build (1 MI355X, % HBM roofline)",
