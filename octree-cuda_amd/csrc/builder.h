@@ -113,6 +113,7 @@ private:
     template <typename T>
     void subsampleLevels();
     void placeInternalNodes();
+    void placeLevelByLevel();
 
     void syncStream();
     template <typename V>

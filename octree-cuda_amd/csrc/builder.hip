@@ -761,7 +761,23 @@ void OctreeBuilder::subsampleLevels()
                         itsNodeGrid.as<NodeGrid>(), ps, bitmap, itsCellOf.as<uint32_t>(), err);
                 OC_LAUNCH_CHECK();
             }
-            scanExclusive<uint32_t>(words, PopcountIn{bitmap}, ExclusiveToArray{prefix},
+            // per-cell state for up to `inputs` cells, reset by the scan for the occupied ones
+            if (!weighted)  // the weighted filter takes its winners from the payload keys
+                itsWin.ensure(sizeof(uint64_t) * inputs);
+            itsCellKey.ensure(sizeof(uint32_t) * inputs);
+            itsCellNode.ensure(sizeof(uint32_t) * inputs);
+            if (packed)
+                itsColor.ensure(sizeof(uint64_t) * inputs);
+            if (weighted)
+            {
+                itsCellCnt.ensure(sizeof(uint32_t) * inputs);
+                itsCellStart.ensure(sizeof(uint32_t) * (inputs + 1));
+                itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(inputs) + 1));
+            }
+            uint64_t* color   = packed ? itsColor.as<uint64_t>() : nullptr;
+            uint32_t* cellCnt = weighted ? itsCellCnt.as<uint32_t>() : nullptr;
+            scanExclusive<uint32_t>(words, PopcountIn{bitmap},
+                                    PrefixClearOut{prefix, weighted ? nullptr : itsWin.as<uint64_t>(), color, cellCnt},
                                     itsScanScratch.as<uint32_t>(), prefix + words, itsStream);
             OC_LAUNCH_CHECK();
             kBatchNodes<<<gridFor(Kb + 1), BLOCK, 0, itsStream>>>(
@@ -775,21 +791,6 @@ void OctreeBuilder::subsampleLevels()
             if (cells == 0)
                 continue;
 
-            itsWin.ensure(sizeof(uint64_t) * cells);
-            itsCellKey.ensure(sizeof(uint32_t) * cells);
-            itsCellNode.ensure(sizeof(uint32_t) * cells);
-            if (packed)
-                itsColor.ensure(sizeof(uint64_t) * cells);
-            if (weighted)
-            {
-                itsCellCnt.ensure(sizeof(uint32_t) * cells);
-                itsCellStart.ensure(sizeof(uint32_t) * (cells + 1));
-                itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(cells) + 1));
-            }
-            uint64_t* color   = packed ? itsColor.as<uint64_t>() : nullptr;
-            uint32_t* cellCnt = weighted ? itsCellCnt.as<uint32_t>() : nullptr;
-            kClearCompact<<<gridFor(cells), BLOCK, 0, itsStream>>>(cells, itsWin.as<uint64_t>(), color, cellCnt);
-            OC_LAUNCH_CHECK();
 
             if (blocks > 0)
             {
@@ -876,10 +877,42 @@ void OctreeBuilder::subsampleLevels()
     itsUsed = cursor;
 }
 
+// Trees up to this many nodes are placed by one workgroup in one launch.
+static constexpr uint32_t kPlaceOneGroupNodes = 1u << 16;
+
 void OctreeBuilder::placeInternalNodes()
 {
     const uint32_t N      = itsCloud.pointAmount;
     const uint64_t staged = itsUsed - N;
+    if (itsM <= kPlaceOneGroupNodes)
+    {
+        PlaceParams pp{};
+        for (uint32_t l = 0; l <= itsDepth + 1 && l <= MAX_LEVELS; ++l)
+            pp.levelStart[l] = itsLevelStart[l];
+        pp.depth = itsDepth;
+        pp.m     = itsM;
+        pp.root  = itsM - 1;
+        pp.N     = N;
+        kPlaceLevels<<<1, PLACE_THREADS, 0, itsStream>>>(pp, itsNodeFlags.as<uint8_t>(), itsNodeChild.as<int32_t>(),
+                                                          itsNodePoints.as<uint32_t>(), itsNodeSub.as<uint32_t>(),
+                                                          itsNodeRegion.as<uint32_t>(), itsNodeData.as<uint64_t>());
+        OC_LAUNCH_CHECK();
+    }
+    else
+        placeLevelByLevel();
+    if (staged > 0)
+    {
+        kPermute<<<gridFor(staged), BLOCK, 0, itsStream>>>(
+                static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
+                itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(), itsLut.as<uint32_t>(),
+                itsOut.as<OutRecord>());
+        OC_LAUNCH_CHECK();
+    }
+}
+
+void OctreeBuilder::placeLevelByLevel()
+{
+    const uint32_t N = itsCloud.pointAmount;
     kFill32<<<gridFor(itsM), BLOCK, 0, itsStream>>>(itsNodeSub.as<uint32_t>(), itsM, 0);
     OC_LAUNCH_CHECK();
     for (uint32_t l = 1; l <= itsDepth; ++l)
@@ -900,14 +933,6 @@ void OctreeBuilder::placeInternalNodes()
         kRegion<<<gridFor(nodes), BLOCK, 0, itsStream>>>(ls, nodes, itsM - 1, N, itsNodeFlags.as<uint8_t>(),
                                                          itsNodeChild.as<int32_t>(), itsNodeSub.as<uint32_t>(),
                                                          itsNodeRegion.as<uint32_t>(), itsNodeData.as<uint64_t>());
-        OC_LAUNCH_CHECK();
-    }
-    if (staged > 0)
-    {
-        kPermute<<<gridFor(staged), BLOCK, 0, itsStream>>>(
-                static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
-                itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(), itsLut.as<uint32_t>(),
-                itsOut.as<OutRecord>());
         OC_LAUNCH_CHECK();
     }
 }

@@ -507,18 +507,29 @@ __global__ __launch_bounds__(BLOCK) void kBatchNodes(uint32_t nodeBase, uint32_t
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __restrict__ win,
-                                                        uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt)
+// Output of the popcount scan: the word's prefix, and the per-cell state of the word's occupied
+// cells (dense indices prefix .. prefix + popcount - 1) reset for the accumulation: no winner,
+// no colour sum, no points. Arrays that the colour mode does not use are null.
+struct PrefixClearOut
 {
-    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    uint32_t* prefix;
+    uint64_t* win;
+    uint64_t* color;
+    uint32_t* cellCnt;
+    __device__ void operator()(uint64_t w, uint32_t p, uint32_t cells) const
     {
-        win[i] = ~0ull;
-        if (color)
-            color[i] = 0;
-        if (cellCnt)
-            cellCnt[i] = 0;
+        prefix[w] = p;
+        for (uint32_t k = 0; k < cells; ++k)
+        {
+            if (win)
+                win[p + k] = ~0ull;
+            if (color)
+                color[p + k] = 0;
+            if (cellCnt)
+                cellCnt[p + k] = 0;
+        }
     }
-}
+};
 
 // Evaluate, part 2: winner (A7/A8: atomicMin of rank << 32 | input position, rank = point index
 // for the implicit method or the keyed bijective hash of it for the explicit one), the packed
@@ -880,7 +891,44 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
 
 // DFS post-order placement (A6, kernel_helpers.cuh:55-59): subtree sums of subsample sizes
 // bottom-up, then region starts top-down; an internal node's data follows its children's
-// subtrees: dataIdx = N + regionStart + sum of its children's subtree sizes.
+// subtrees: dataIdx = N + regionStart + sum of its children's subtree sizes. kSubtree / kRegion
+// take one level per launch; kPlaceLevels does all levels in one workgroup for small trees (a
+// small build is bound by its launch count, not by these few thousand nodes).
+__device__ __forceinline__ void subtreeOf(uint32_t id, const uint8_t* __restrict__ flags,
+                                          const int32_t* __restrict__ children,
+                                          const uint32_t* __restrict__ pointCount, uint32_t* __restrict__ subtree)
+{
+    if (!(flags[id] & NODE_INTERNAL))
+        return;
+    uint32_t s = pointCount[id];
+    for (int k = 0; k < 8; ++k)
+    {
+        const int32_t c = children[8 * static_cast<uint64_t>(id) + k];
+        if (c >= 0)
+            s += subtree[c];
+    }
+    subtree[id] = s;
+}
+
+__device__ __forceinline__ void regionOf(uint32_t id, uint32_t root, uint32_t N, const uint8_t* __restrict__ flags,
+                                         const int32_t* __restrict__ children, const uint32_t* __restrict__ subtree,
+                                         uint32_t* __restrict__ region, uint64_t* __restrict__ dataIdx)
+{
+    if (!(flags[id] & NODE_INTERNAL))
+        return;
+    uint32_t r = (id == root) ? 0u : region[id];
+    for (int k = 0; k < 8; ++k)
+    {
+        const int32_t c = children[8 * static_cast<uint64_t>(id) + k];
+        if (c >= 0)
+        {
+            region[c] = r;
+            r += subtree[c];
+        }
+    }
+    dataIdx[id] = static_cast<uint64_t>(N) + r;
+}
+
 __global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t nodes,
                                                    const uint8_t* __restrict__ flags,
                                                    const int32_t* __restrict__ children,
@@ -888,19 +936,7 @@ __global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t 
                                                    uint32_t* __restrict__ subtree)
 {
     for (uint64_t j = globalThread(); j < nodes; j += globalStride())
-    {
-        uint32_t id = levelStart + static_cast<uint32_t>(j);
-        if (!(flags[id] & NODE_INTERNAL))
-            continue;
-        uint32_t s = pointCount[id];
-        for (int k = 0; k < 8; ++k)
-        {
-            int32_t c = children[8 * static_cast<uint64_t>(id) + k];
-            if (c >= 0)
-                s += subtree[c];
-        }
-        subtree[id] = s;
-    }
+        subtreeOf(levelStart + static_cast<uint32_t>(j), flags, children, pointCount, subtree);
 }
 
 __global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t nodes, uint32_t root, uint32_t N,
@@ -910,21 +946,40 @@ __global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t n
                                                   uint32_t* __restrict__ region, uint64_t* __restrict__ dataIdx)
 {
     for (uint64_t j = globalThread(); j < nodes; j += globalStride())
+        regionOf(levelStart + static_cast<uint32_t>(j), root, N, flags, children, subtree, region, dataIdx);
+}
+
+constexpr int PLACE_THREADS = 1024;
+
+struct PlaceParams
+{
+    uint32_t levelStart[MAX_LEVELS + 1];  // sparse ids of level l: [levelStart[l], levelStart[l + 1])
+    uint32_t depth, m, root, N;
+};
+
+// One workgroup: subtree sizes zeroed, then summed level by level bottom-up, then regions and data
+// offsets top-down, a barrier between levels (the workgroup's writes are visible to it after one).
+__global__ __launch_bounds__(PLACE_THREADS) void kPlaceLevels(PlaceParams pp, const uint8_t* __restrict__ flags,
+                                                              const int32_t* __restrict__ children,
+                                                              const uint32_t* __restrict__ pointCount,
+                                                              uint32_t* __restrict__ subtree,
+                                                              uint32_t* __restrict__ region,
+                                                              uint64_t* __restrict__ dataIdx)
+{
+    for (uint32_t i = threadIdx.x; i < pp.m; i += PLACE_THREADS)
+        subtree[i] = 0;
+    __syncthreads();
+    for (uint32_t l = 1; l <= pp.depth; ++l)
     {
-        uint32_t id = levelStart + static_cast<uint32_t>(j);
-        if (!(flags[id] & NODE_INTERNAL))
-            continue;
-        uint32_t r = (id == root) ? 0u : region[id];
-        for (int k = 0; k < 8; ++k)
-        {
-            int32_t c = children[8 * static_cast<uint64_t>(id) + k];
-            if (c >= 0)
-            {
-                region[c] = r;
-                r += subtree[c];
-            }
-        }
-        dataIdx[id] = static_cast<uint64_t>(N) + r;
+        for (uint32_t id = pp.levelStart[l] + threadIdx.x; id < pp.levelStart[l + 1]; id += PLACE_THREADS)
+            subtreeOf(id, flags, children, pointCount, subtree);
+        __syncthreads();
+    }
+    for (uint32_t l = pp.depth; l >= 1; --l)
+    {
+        for (uint32_t id = pp.levelStart[l] + threadIdx.x; id < pp.levelStart[l + 1]; id += PLACE_THREADS)
+            regionOf(id, pp.root, pp.N, flags, children, subtree, region, dataIdx);
+        __syncthreads();
     }
 }
 
