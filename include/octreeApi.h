@@ -155,7 +155,9 @@ EXPORTED int ocpi_service_set_active_slots (void* service, uint32_t slots);
 /* stats: [0] submitted [1] completed [2] failed [3] queued now [4] running now [5] max queue depth
  * [6] p50 latency ms [7] p99 latency ms [8] mean latency ms [9] reserved device bytes
  * [10] memory budget [11] active slots [12] slots [13] mean queue wait ms [14] max latency ms
- * [15] peak reserved device bytes. Latency = submit to result attached to the session. */
+ * [15] peak reserved device bytes [16] [17] [18] mean time per completed request in setup (arena
+ * sizing and cloud upload), generate and result capture, ms. Latency = submit to result attached
+ * to the session. */
 EXPORTED int ocpi_service_get_stats (void* service, double* stats, uint32_t n);
 EXPORTED int ocpi_service_reset_stats (void* service);
 /* Of the session's last service request: [0] queue wait ms [1] build + copy ms [2] total ms. */

@@ -2,38 +2,78 @@
 
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 
 #include "pinned_pool.h"
 
 namespace oc {
 
+uint64_t BuildResult::snapshotBytes() const
+{
+    return SnapshotLayout(sparseNodes(), used()).bytes;
+}
+
+void BuildResult::copySnapshot(uint8_t* dst) const
+{
+    const SnapshotLayout l(sparseNodes(), used());
+    HostOctree o;
+    copyNodes(o);
+    const uint64_t m = sparseNodes();
+    std::memcpy(dst + l.dataIdx, o.dataIdx.data(), 8 * m);
+    std::memcpy(dst + l.denseKey, o.denseKey.data(), 4 * m);
+    std::memcpy(dst + l.count, o.count.data(), 4 * m);
+    std::memcpy(dst + l.pointCount, o.pointCount.data(), 4 * m);
+    std::memcpy(dst + l.parent, o.parent.data(), 4 * m);
+    std::memcpy(dst + l.children, o.children.data(), 32 * m);
+    std::memcpy(dst + l.flags, o.flags.data(), m);
+    if (used())
+    {
+        copyLut(reinterpret_cast<uint32_t*>(dst + l.lut), used());
+        copyOutputRange(dst + l.out, 0, used());
+    }
+}
+
 std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& source, PinnedPool* pool)
 {
-    auto r       = std::unique_ptr<HostBuildResult>(new HostBuildResult());
-    r->itsCloud  = source.cloud();
-    r->itsProc   = source.processing();
-    r->itsDepth  = source.depth();
-    r->itsDense  = source.denseNodes();
-    r->itsM      = source.sparseNodes();
-    r->itsE      = source.capacity();
-    r->itsUsed   = source.used();
+    auto r        = std::unique_ptr<HostBuildResult>(new HostBuildResult());
+    r->itsCloud   = source.cloud();
+    r->itsProc    = source.processing();
+    r->itsDepth   = source.depth();
+    r->itsDense   = source.denseNodes();
+    r->itsM       = source.sparseNodes();
+    r->itsE       = source.capacity();
+    r->itsUsed    = source.used();
     r->itsTimings = source.stageTimings();
-    source.copyNodes(r->itsNodes);
-    const size_t lutBytes = r->itsUsed * sizeof(uint32_t);
-    const size_t bytes    = lutBytes + r->itsUsed * sizeof(OutRecord) + 1;
+    const SnapshotLayout l(r->itsM, r->itsUsed);
+    const uint64_t bytes = l.bytes + 1;
     if (pool)
         r->itsBlock = pool->acquire(bytes);
     else
         r->itsBlock = std::shared_ptr<uint8_t>(new uint8_t[bytes], std::default_delete<uint8_t[]>());
     uint8_t* base = r->itsBlock.get();
-    r->itsLut     = reinterpret_cast<const uint32_t*>(base);
-    r->itsOut     = base + lutBytes;
-    if (r->itsUsed)
-    {
-        source.copyLut(reinterpret_cast<uint32_t*>(base), r->itsUsed);
-        source.copyOutputRange(base + lutBytes, 0, r->itsUsed);
-    }
+    source.copySnapshot(base);
+    r->itsLut = reinterpret_cast<const uint32_t*>(base + l.lut);
+    r->itsOut = base + l.out;
     return r;
+}
+
+void HostBuildResult::copyNodes(HostOctree& o) const
+{
+    const SnapshotLayout l(itsM, itsUsed);
+    const uint8_t* base = itsBlock.get();
+    const uint64_t m    = itsM;
+    auto take = [&](auto& v, uint64_t offset, uint64_t n) {
+        using V = typename std::decay_t<decltype(v)>::value_type;
+        v.resize(n);
+        std::memcpy(v.data(), base + offset, n * sizeof(V));
+    };
+    take(o.dataIdx, l.dataIdx, m);
+    take(o.denseKey, l.denseKey, m);
+    take(o.count, l.count, m);
+    take(o.pointCount, l.pointCount, m);
+    take(o.parent, l.parent, m);
+    take(o.children, l.children, 8 * m);
+    take(o.flags, l.flags, m);
 }
 
 void HostBuildResult::copyLut(uint32_t* dst, uint64_t count) const

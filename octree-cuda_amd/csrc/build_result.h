@@ -63,6 +63,31 @@ public:
     virtual void copyLut(uint32_t* dst, uint64_t count) const                         = 0;
     virtual void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const = 0;
     void copyOutput(uint8_t* dst, uint64_t count) const { copyOutputRange(dst, 0, count); }
+
+    // Everything a host snapshot holds, packed into one buffer of snapshotBytes() bytes (layout:
+    // SnapshotLayout). A device builder issues all of its copies before a single wait.
+    virtual void copySnapshot(uint8_t* dst) const;
+    uint64_t snapshotBytes() const;
+};
+
+// Packing of a snapshot: the node arrays (dataIdx first, for its 8-byte alignment), then the used
+// LUT entries and output records.
+struct SnapshotLayout
+{
+    uint64_t dataIdx, denseKey, count, pointCount, parent, children, flags, lut, out, bytes;
+    SnapshotLayout(uint32_t m, uint64_t used)
+    {
+        dataIdx    = 0;
+        denseKey   = dataIdx + 8ull * m;
+        count      = denseKey + 4ull * m;
+        pointCount = count + 4ull * m;
+        parent     = pointCount + 4ull * m;
+        children   = parent + 4ull * m;
+        flags      = children + 32ull * m;
+        lut        = (flags + m + 3) & ~3ull;
+        out        = lut + 4 * used;
+        bytes      = out + 18 * used;
+    }
 };
 
 // Snapshot of a finished build in host memory: the nodes and the used part of the LUT and the
@@ -83,7 +108,7 @@ public:
     uint64_t used() const override { return itsUsed; }
     const std::vector<float>& stageTimings() const override { return itsTimings; }
 
-    void copyNodes(HostOctree& out) const override { out = itsNodes; }
+    void copyNodes(HostOctree& out) const override;
     void copyLut(uint32_t* dst, uint64_t count) const override;
     void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
 
@@ -93,8 +118,7 @@ private:
     uint32_t itsDepth = 0, itsDense = 0, itsM = 0;
     uint64_t itsE = 0, itsUsed = 0;
     std::vector<float> itsTimings;
-    HostOctree itsNodes;
-    std::shared_ptr<uint8_t> itsBlock;  // the LUT (used entries) followed by the output records
+    std::shared_ptr<uint8_t> itsBlock;  // the snapshot (SnapshotLayout)
     const uint32_t* itsLut = nullptr;
     const uint8_t* itsOut  = nullptr;
 };

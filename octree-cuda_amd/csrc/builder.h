@@ -99,6 +99,7 @@ public:
     void copyNodes(HostOctree& out) const override;
     void copyLut(uint32_t* dst, uint64_t count) const override;
     void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
+    void copySnapshot(uint8_t* dst) const override;
 
 private:
     template <typename T>
@@ -118,11 +119,13 @@ private:
     void syncStream();
     template <typename V>
     V readDevice(const V* devicePtr);
+    std::array<uint64_t, 2> readDevicePair(const uint64_t* devicePtr);
+    void waitMailbox(uint64_t seq);
 
     int itsDevice;
     hipStream_t itsStream = nullptr;
-    // Size readbacks: a one-thread kernel publishes {value, sequence} into coherent pinned host
-    // memory and the host spins on the sequence (no staging copy, no blocking-sync wake-up).
+    // Size readbacks: a one-thread kernel publishes {value, value, sequence} into coherent pinned
+    // host memory and the host spins on the sequence (no staging copy, no blocking-sync wake-up).
     uint64_t* itsMailbox  = nullptr;
     uint64_t itsMailSeq   = 0;
     bool itsReady         = false;  // setup() has run

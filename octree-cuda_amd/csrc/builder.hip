@@ -121,9 +121,10 @@ OctreeBuilder::OctreeBuilder(int device) : itsDevice(device)
     OC_HIP_CHECK(hipStreamCreateWithFlags(&itsStream, hipStreamNonBlocking));
     for (auto& ev : itsEvents)
         OC_HIP_CHECK(hipEventCreate(&ev));
-    OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&itsMailbox), 2 * sizeof(uint64_t), hipHostMallocCoherent));
+    OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&itsMailbox), 3 * sizeof(uint64_t), hipHostMallocCoherent));
     itsMailbox[0] = 0;
     itsMailbox[1] = 0;
+    itsMailbox[2] = 0;
 }
 
 OctreeBuilder::OctreeBuilder(int device, const CloudInfo& cloud, const ProcessingInfo& proc,
@@ -312,32 +313,56 @@ __global__ void kPublish(const V* __restrict__ src, uint64_t* box, uint64_t seq)
     if (threadIdx.x != 0)
         return;
     __hip_atomic_store(box, static_cast<uint64_t>(*src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(box + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(box + 2, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Two consecutive device words through the mailbox (one wait for two sizes).
+__global__ void kPublishPair(const uint64_t* __restrict__ src, uint64_t* box, uint64_t seq)
+{
+    if (threadIdx.x != 0)
+        return;
+    __hip_atomic_store(box, src[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(box + 1, src[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(box + 2, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A size the host needs to size the next launches. The stream is drained up to the publish: the
 // host spins on the mailbox for up to 2 ms (the common case: the value is ready within microseconds
 // of the last kernel) and then falls back to a blocking synchronisation, which also reports a
 // failed kernel instead of waiting forever.
+void OctreeBuilder::waitMailbox(uint64_t seq)
+{
+    const auto start = std::chrono::steady_clock::now();
+    while (__atomic_load_n(itsMailbox + 2, __ATOMIC_ACQUIRE) != seq)
+    {
+        __builtin_ia32_pause();
+        if (std::chrono::steady_clock::now() - start > std::chrono::milliseconds(2))
+        {
+            OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+            if (__atomic_load_n(itsMailbox + 2, __ATOMIC_ACQUIRE) != seq)
+                throw std::runtime_error("device value was not published");
+            break;
+        }
+    }
+}
+
 template <typename V>
 V OctreeBuilder::readDevice(const V* devicePtr)
 {
     const uint64_t seq = ++itsMailSeq;
     kPublish<V><<<1, 64, 0, itsStream>>>(devicePtr, itsMailbox, seq);
     OC_LAUNCH_CHECK();
-    const auto start = std::chrono::steady_clock::now();
-    while (__atomic_load_n(itsMailbox + 1, __ATOMIC_ACQUIRE) != seq)
-    {
-        __builtin_ia32_pause();
-        if (std::chrono::steady_clock::now() - start > std::chrono::milliseconds(2))
-        {
-            OC_HIP_CHECK(hipStreamSynchronize(itsStream));
-            if (__atomic_load_n(itsMailbox + 1, __ATOMIC_ACQUIRE) != seq)
-                throw std::runtime_error("device value was not published");
-            break;
-        }
-    }
+    waitMailbox(seq);
     return static_cast<V>(__atomic_load_n(itsMailbox, __ATOMIC_ACQUIRE));
+}
+
+std::array<uint64_t, 2> OctreeBuilder::readDevicePair(const uint64_t* devicePtr)
+{
+    const uint64_t seq = ++itsMailSeq;
+    kPublishPair<<<1, 64, 0, itsStream>>>(devicePtr, itsMailbox, seq);
+    OC_LAUNCH_CHECK();
+    waitMailbox(seq);
+    return {__atomic_load_n(itsMailbox, __ATOMIC_ACQUIRE), __atomic_load_n(itsMailbox + 1, __ATOMIC_ACQUIRE)};
 }
 
 void OctreeBuilder::generate()
@@ -683,39 +708,65 @@ void OctreeBuilder::subsampleLevels()
         sp.levelGrid   = itsLevelGrid[l];
         sp.side        = cubicSize / std::pow(2.0, static_cast<double>(itsDepth - l));  // rp_subsampling.cu:17-18
 
-        // internal nodes of the level
+        // the internal nodes of the level, their input segments and griddings, the segment scan
+        // and the tile list: one workgroup and one readback for small levels, four launches and
+        // two readbacks otherwise (inputs <= cursor: leaf points plus everything subsampled so far)
+        const bool smallLevel = levelNodes <= SMALL_LEVEL_NODES && cursor <= SMALL_LEVEL_POINTS;
         itsInternalList.ensure(sizeof(uint32_t) * levelNodes);
-        itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(8ull * levelNodes) + 1));
-        scanExclusive<uint32_t>(levelNodes, InternalIn{itsNodeFlags.as<uint8_t>(), ls},
-                                InternalOut{itsInternalList.as<uint32_t>(), ls}, itsScanScratch.as<uint32_t>(),
-                                reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL), itsStream);
-        OC_LAUNCH_CHECK();
-        const uint32_t K = static_cast<uint32_t>(readDevice(sc + SC_TABLE_TOTAL));
-        if (K == 0)
-            continue;
-        const uint64_t segments = 8ull * K;
-        itsSegCount.ensure(sizeof(uint32_t) * segments);
-        itsSegSrc.ensure(sizeof(uint32_t) * segments);
-        itsSegTile.ensure(sizeof(uint32_t) * segments);
-        itsSegInput.ensure(sizeof(uint32_t) * segments);
-        itsNodeGrid.ensure(sizeof(NodeGrid) * K);
-        kSegments<<<gridFor(K), BLOCK, 0, itsStream>>>(
-                K, sp, itsInternalList.as<uint32_t>(), itsNodeKey.as<uint32_t>(), itsNodeFlags.as<uint8_t>(),
-                itsNodeChild.as<int32_t>(), itsNodePoints.as<uint32_t>(), itsNodeSorted.as<uint32_t>(),
-                itsNodeStage.as<uint32_t>(), itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(),
-                itsNodeGrid.as<NodeGrid>());
-        OC_LAUNCH_CHECK();
-        scanExclusive<uint64_t>(segments, SegmentIn{itsSegCount.as<uint32_t>()},
-                                SegmentOut{itsSegTile.as<uint32_t>(), itsSegInput.as<uint32_t>()},
-                                itsScanScratch.as<uint64_t>(), sc + SC_SEG_TOTAL, itsStream);
-        OC_LAUNCH_CHECK();
-        const uint64_t segTotal = readDevice(sc + SC_SEG_TOTAL);
-        const uint32_t tiles    = static_cast<uint32_t>(segTotal >> 32);
-        const uint32_t inputs   = static_cast<uint32_t>(segTotal);
+        itsSegCount.ensure(sizeof(uint32_t) * 8 * levelNodes);
+        itsSegSrc.ensure(sizeof(uint32_t) * 8 * levelNodes);
+        itsSegTile.ensure(sizeof(uint32_t) * 8 * levelNodes);
+        itsSegInput.ensure(sizeof(uint32_t) * 8 * levelNodes);
+        itsNodeGrid.ensure(sizeof(NodeGrid) * levelNodes);
+        auto tables = [&]() {
+            return LevelTables{itsNodeKey.as<uint32_t>(),     itsNodeFlags.as<uint8_t>(),   itsNodeChild.as<int32_t>(),
+                               itsNodePoints.as<uint32_t>(),  itsNodeSorted.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
+                               itsInternalList.as<uint32_t>(), itsSegCount.as<uint32_t>(),  itsSegSrc.as<uint32_t>(),
+                               itsSegTile.as<uint32_t>(),     itsSegInput.as<uint32_t>(),   itsTileList.as<uint32_t>(),
+                               itsNodeGrid.as<NodeGrid>()};
+        };
+        uint32_t K        = 0;
+        uint64_t segTotal = 0;
+        if (smallLevel)
+        {
+            itsTileList.ensure(sizeof(uint32_t) * (cursor / TILE + 8ull * levelNodes + 1));
+            kLevelSetup<<<1, SCAN_THREADS, 0, itsStream>>>(ls, levelNodes, sp, tables(),
+                                                           reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL),
+                                                           sc + SC_SEG_TOTAL);
+            OC_LAUNCH_CHECK();
+            const auto v = readDevicePair(sc + SC_SEG_TOTAL);  // segment total, internal-node count
+            segTotal     = v[0];
+            K            = static_cast<uint32_t>(v[1]);
+            if (K == 0)
+                continue;
+        }
+        else
+        {
+            itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(8ull * levelNodes) + 1));
+            scanExclusive<uint32_t>(levelNodes, InternalIn{itsNodeFlags.as<uint8_t>(), ls},
+                                    InternalOut{itsInternalList.as<uint32_t>(), ls}, itsScanScratch.as<uint32_t>(),
+                                    reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL), itsStream);
+            OC_LAUNCH_CHECK();
+            K = static_cast<uint32_t>(readDevice(sc + SC_TABLE_TOTAL));
+            if (K == 0)
+                continue;
+            kSegments<<<gridFor(K), BLOCK, 0, itsStream>>>(K, sp, tables());
+            OC_LAUNCH_CHECK();
+            scanExclusive<uint64_t>(8ull * K, SegmentIn{itsSegCount.as<uint32_t>()},
+                                    SegmentOut{itsSegTile.as<uint32_t>(), itsSegInput.as<uint32_t>()},
+                                    itsScanScratch.as<uint64_t>(), sc + SC_SEG_TOTAL, itsStream);
+            OC_LAUNCH_CHECK();
+            segTotal = readDevice(sc + SC_SEG_TOTAL);
+            itsTileList.ensure(sizeof(uint32_t) * (segTotal >> 32));
+            kTileList<<<gridFor(8ull * K), BLOCK, 0, itsStream>>>(8ull * K, itsSegCount.as<uint32_t>(),
+                                                                  itsSegTile.as<uint32_t>(), itsTileList.as<uint32_t>());
+            OC_LAUNCH_CHECK();
+        }
+        const uint32_t tiles  = static_cast<uint32_t>(segTotal >> 32);
+        const uint32_t inputs = static_cast<uint32_t>(segTotal);
         if (inputs == 0)
             continue;
 
-        itsTileList.ensure(sizeof(uint32_t) * tiles);
         itsCellOf.ensure(sizeof(uint32_t) * inputs);
         if (weighted)
         {
@@ -725,9 +776,6 @@ void OctreeBuilder::subsampleLevels()
         }
         uint64_t* winKey =
                 weighted ? reinterpret_cast<uint64_t*>(itsPayload.as<uint8_t>() + sizeof(WeightPayload) * inputs) : nullptr;
-        kTileList<<<gridFor(segments), BLOCK, 0, itsStream>>>(segments, itsSegCount.as<uint32_t>(),
-                                                              itsSegTile.as<uint32_t>(), itsTileList.as<uint32_t>());
-        OC_LAUNCH_CHECK();
 
         for (uint32_t k0 = 0; k0 < K; k0 += batchNodes)
         {
@@ -761,24 +809,34 @@ void OctreeBuilder::subsampleLevels()
                         itsNodeGrid.as<NodeGrid>(), ps, bitmap, itsCellOf.as<uint32_t>(), err);
                 OC_LAUNCH_CHECK();
             }
-            // per-cell state for up to `inputs` cells, reset by the scan for the occupied ones
-            if (!weighted)  // the weighted filter takes its winners from the payload keys
-                itsWin.ensure(sizeof(uint64_t) * inputs);
-            itsCellKey.ensure(sizeof(uint32_t) * inputs);
-            itsCellNode.ensure(sizeof(uint32_t) * inputs);
-            if (packed)
-                itsColor.ensure(sizeof(uint64_t) * inputs);
-            if (weighted)
+            // per-cell state, reset for the occupied cells: by the scan itself on small levels
+            // (arrays sized by the inputs, an upper bound of the cells), else by kClearCompact
+            auto sizeCells = [&](uint64_t n) {
+                if (!weighted)  // the weighted filter takes its winners from the payload keys
+                    itsWin.ensure(sizeof(uint64_t) * n);
+                itsCellKey.ensure(sizeof(uint32_t) * n);
+                itsCellNode.ensure(sizeof(uint32_t) * n);
+                if (packed)
+                    itsColor.ensure(sizeof(uint64_t) * n);
+                if (weighted)
+                {
+                    itsCellCnt.ensure(sizeof(uint32_t) * n);
+                    itsCellStart.ensure(sizeof(uint32_t) * (n + 1));
+                    itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(n) + 1));
+                }
+            };
+            if (smallLevel)
             {
-                itsCellCnt.ensure(sizeof(uint32_t) * inputs);
-                itsCellStart.ensure(sizeof(uint32_t) * (inputs + 1));
-                itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(inputs) + 1));
+                sizeCells(inputs);
+                scanExclusive<uint32_t>(words, PopcountIn{bitmap},
+                                        PrefixClearOut{prefix, weighted ? nullptr : itsWin.as<uint64_t>(),
+                                                       packed ? itsColor.as<uint64_t>() : nullptr,
+                                                       weighted ? itsCellCnt.as<uint32_t>() : nullptr},
+                                        itsScanScratch.as<uint32_t>(), prefix + words, itsStream);
             }
-            uint64_t* color   = packed ? itsColor.as<uint64_t>() : nullptr;
-            uint32_t* cellCnt = weighted ? itsCellCnt.as<uint32_t>() : nullptr;
-            scanExclusive<uint32_t>(words, PopcountIn{bitmap},
-                                    PrefixClearOut{prefix, weighted ? nullptr : itsWin.as<uint64_t>(), color, cellCnt},
-                                    itsScanScratch.as<uint32_t>(), prefix + words, itsStream);
+            else
+                scanExclusive<uint32_t>(words, PopcountIn{bitmap}, ExclusiveToArray{prefix},
+                                        itsScanScratch.as<uint32_t>(), prefix + words, itsStream);
             OC_LAUNCH_CHECK();
             kBatchNodes<<<gridFor(Kb + 1), BLOCK, 0, itsStream>>>(
                     k0, Kb, wordsPerNode, itsNodeGrid.as<NodeGrid>(), prefix, static_cast<uint32_t>(cursor),
@@ -790,6 +848,16 @@ void OctreeBuilder::subsampleLevels()
                                          "raise the output factor");
             if (cells == 0)
                 continue;
+            if (!smallLevel)
+            {
+                sizeCells(cells);
+                kClearCompact<<<gridFor(cells), BLOCK, 0, itsStream>>>(
+                        cells, weighted ? nullptr : itsWin.as<uint64_t>(), packed ? itsColor.as<uint64_t>() : nullptr,
+                        weighted ? itsCellCnt.as<uint32_t>() : nullptr);
+                OC_LAUNCH_CHECK();
+            }
+            uint64_t* color   = packed ? itsColor.as<uint64_t>() : nullptr;
+            uint32_t* cellCnt = weighted ? itsCellCnt.as<uint32_t>() : nullptr;
 
 
             if (blocks > 0)
@@ -960,6 +1028,28 @@ void OctreeBuilder::copyNodes(HostOctree& o) const
     copy(o.parent.data(), itsNodeParent.get(), 4 * m);
     copy(o.children.data(), itsNodeChild.get(), 32 * m);
     copy(o.flags.data(), itsNodeFlags.get(), m);
+    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+}
+
+void OctreeBuilder::copySnapshot(uint8_t* dst) const
+{
+    if (!itsBuilt)
+        throw std::runtime_error("the octree has not been generated");
+    const SnapshotLayout l(itsM, itsUsed);
+    const uint64_t m = itsM;
+    auto copy = [&](uint64_t offset, const void* src, uint64_t bytes) {
+        if (bytes)
+            OC_HIP_CHECK(hipMemcpyAsync(dst + offset, src, bytes, hipMemcpyDeviceToHost, itsStream));
+    };
+    copy(l.dataIdx, itsNodeData.get(), 8 * m);
+    copy(l.denseKey, itsNodeKey.get(), 4 * m);
+    copy(l.count, itsNodeCount.get(), 4 * m);
+    copy(l.pointCount, itsNodePoints.get(), 4 * m);
+    copy(l.parent, itsNodeParent.get(), 4 * m);
+    copy(l.children, itsNodeChild.get(), 32 * m);
+    copy(l.flags, itsNodeFlags.get(), m);
+    copy(l.lut, itsLut.get(), 4 * itsUsed);
+    copy(l.out, itsOut.get(), sizeof(OutRecord) * itsUsed);
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 

@@ -291,6 +291,28 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
     const uint32_t b     = blockIdx.x;
     const uint32_t cells = 1u << (3 * fp.lc);
     const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
+    const uint32_t bx = compact3(b >> 2), by = compact3(b >> 1), bz = compact3(b);
+
+    if (lo == hi)
+    {
+        // an empty bucket (most of them in a sparse cloud): zero counts on levels 0..lc, no
+        // histogram and no scan
+        for (uint32_t k = 0; k <= fp.lc; ++k)
+        {
+            const uint32_t side   = 1u << (fp.lc - k);
+            const uint32_t n3     = side * side * side;
+            const uint64_t gk     = fp.levelGrid[k];
+            const uint64_t offset = fp.levelOffset[k];
+            for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
+            {
+                const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
+                const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
+                               gz = static_cast<uint64_t>(bz) * side + lz;
+                count[offset + gx + gy * gk + gz * gk * gk] = 0;
+            }
+        }
+        return;
+    }
 
     for (uint32_t c = threadIdx.x; c <= cells; c += SORT_THREADS)
         h[c] = 0;
@@ -316,7 +338,6 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
 
     // dense counts of this bucket's cells on levels 0..lc (kernelPropagatePointCounts,
     // point_count_propagation.cuh:34-86, as differences of the Morton prefix)
-    const uint32_t bx = compact3(b >> 2), by = compact3(b >> 1), bz = compact3(b);
     for (uint32_t k = 0; k <= fp.lc; ++k)
     {
         const uint32_t side   = 1u << (fp.lc - k);

@@ -164,42 +164,53 @@ __device__ __forceinline__ Gridding nodeGridding(uint32_t lin, uint32_t gl, uint
 
 // Per internal node k of the level: its eight input segments (children in slot order, the leaf's
 // range of the sorted cloud or the child's staged subsample) and its gridding (nodeGridding).
-__global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp,
-                                                    const uint32_t* __restrict__ internalList,
-                                                    const uint32_t* __restrict__ nodeKey,
-                                                    const uint8_t* __restrict__ flags,
-                                                    const int32_t* __restrict__ children,
-                                                    const uint32_t* __restrict__ pointCount,
-                                                    const uint32_t* __restrict__ sortedStart,
-                                                    const uint32_t* __restrict__ stage,
-                                                    uint32_t* __restrict__ segCount, uint32_t* __restrict__ segSrc,
-                                                    NodeGrid* __restrict__ nodeGrid)
+struct LevelTables
+{
+    const uint32_t* nodeKey;
+    const uint8_t* flags;
+    const int32_t* children;
+    const uint32_t* pointCount;
+    const uint32_t* sortedStart;
+    const uint32_t* stage;
+    uint32_t* internalList;
+    uint32_t* segCount;
+    uint32_t* segSrc;
+    uint32_t* segTile;
+    uint32_t* segInput;
+    uint32_t* tileList;
+    NodeGrid* nodeGrid;
+};
+
+__device__ __forceinline__ void segmentsOf(uint64_t k, const SubsampleParams& sp, const LevelTables& lt)
+{
+    const uint32_t id = lt.internalList[k];
+    for (int s = 0; s < 8; ++s)
+    {
+        const int32_t c = lt.children[8 * static_cast<uint64_t>(id) + s];
+        uint32_t cnt = 0, src = 0;
+        if (c >= 0)
+        {
+            cnt = lt.pointCount[c];
+            src = (lt.flags[c] & NODE_INTERNAL) ? lt.stage[c] : lt.sortedStart[c];
+        }
+        lt.segCount[8 * k + s] = cnt;
+        lt.segSrc[8 * k + s]   = src;
+    }
+    const uint32_t key = lt.nodeKey[id];
+    NodeGrid ng;
+    ng.grid        = nodeGridding(key - sp.levelOffset, sp.levelGrid, sp.level, sp.depth, sp.cubicMin, sp.sideHalf,
+                                  sp.side, sp.g);
+    const double t = ng.grid.bbSize / sp.g;
+    ng.diag        = sqrt((t * 3) * (t * 3) * 3) / 2.0;
+    ng.salt        = nodeSalt(sp.seed, key);
+    ng.id          = id;
+    lt.nodeGrid[k] = ng;
+}
+
+__global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp, LevelTables lt)
 {
     for (uint64_t k = globalThread(); k < nodes; k += globalStride())
-    {
-        uint32_t id = internalList[k];
-        for (int s = 0; s < 8; ++s)
-        {
-            int32_t c    = children[8 * static_cast<uint64_t>(id) + s];
-            uint32_t cnt = 0, src = 0;
-            if (c >= 0)
-            {
-                cnt = pointCount[c];
-                src = (flags[c] & NODE_INTERNAL) ? stage[c] : sortedStart[c];
-            }
-            segCount[8 * k + s] = cnt;
-            segSrc[8 * k + s]   = src;
-        }
-        const uint32_t key = nodeKey[id];
-        NodeGrid ng;
-        ng.grid          = nodeGridding(key - sp.levelOffset, sp.levelGrid, sp.level, sp.depth, sp.cubicMin,
-                                        sp.sideHalf, sp.side, sp.g);
-        double t         = ng.grid.bbSize / sp.g;
-        ng.diag          = sqrt((t * 3) * (t * 3) * 3) / 2.0;
-        ng.salt          = nodeSalt(sp.seed, key);
-        ng.id            = id;
-        nodeGrid[k]      = ng;
-    }
+        segmentsOf(k, sp, lt);
 }
 
 // The sorted cloud in leaf and cell order. The fine sort's permutation puts the points in Morton
@@ -408,6 +419,39 @@ __global__ __launch_bounds__(BLOCK) void kTileList(uint64_t segments, const uint
     }
 }
 
+// A whole level's setup in one workgroup and one launch, for small levels (up to
+// SMALL_LEVEL_NODES nodes and SMALL_LEVEL_POINTS candidate inputs; a small build is bound by its
+// launch count): the internal nodes of the level, their segments and griddings, the segment scan
+// and the tile list. Writes K to *internalTotal and (tiles << 32 | inputs) to *segTotal.
+constexpr uint32_t SMALL_LEVEL_NODES  = 1u << 16;
+constexpr uint64_t SMALL_LEVEL_POINTS = 1u << 22;
+
+__global__ __launch_bounds__(SCAN_THREADS) void kLevelSetup(uint32_t levelStart, uint32_t levelNodes,
+                                                            SubsampleParams sp, LevelTables lt,
+                                                            uint32_t* __restrict__ internalTotal,
+                                                            uint64_t* __restrict__ segTotal)
+{
+    __shared__ uint32_t lds32[SCAN_WAVES];
+    __shared__ uint64_t lds64[SCAN_WAVES];
+    blockScanAll<uint32_t>(InternalIn{lt.flags, levelStart}, InternalOut{lt.internalList, levelStart}, levelNodes,
+                           internalTotal, lds32);
+    __syncthreads();
+    const uint32_t K = *internalTotal;
+    for (uint32_t k = threadIdx.x; k < K; k += SCAN_THREADS)
+        segmentsOf(k, sp, lt);
+    __syncthreads();
+    const uint64_t segments = 8ull * K;
+    blockScanAll<uint64_t>(SegmentIn{lt.segCount}, SegmentOut{lt.segTile, lt.segInput}, segments, segTotal, lds64);
+    __syncthreads();
+    for (uint64_t s = threadIdx.x; s < segments; s += SCAN_THREADS)
+    {
+        const uint32_t tiles = (lt.segCount[s] + TILE - 1) / TILE;
+        const uint32_t first = lt.segTile[s];
+        for (uint32_t t = 0; t < tiles; ++t)
+            lt.tileList[first + t] = static_cast<uint32_t>(s);
+    }
+}
+
 // Identifies the input of this thread: tile -> segment -> (node k, input position, input index q).
 struct InputRef
 {
@@ -507,7 +551,23 @@ __global__ __launch_bounds__(BLOCK) void kBatchNodes(uint32_t nodeBase, uint32_t
     }
 }
 
-// Output of the popcount scan: the word's prefix, and the per-cell state of the word's occupied
+__global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __restrict__ win,
+                                                        uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt)
+{
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        if (win)
+            win[i] = ~0ull;
+        if (color)
+            color[i] = 0;
+        if (cellCnt)
+            cellCnt[i] = 0;
+    }
+}
+
+// Output of the popcount scan for small levels (one launch less than a separate clear; on large
+// levels the per-word loops cost more than kClearCompact): the word's prefix, and the per-cell
+// state of the word's occupied
 // cells (dense indices prefix .. prefix + popcount - 1) reset for the accumulation: no winner,
 // no colour sum, no points. Arrays that the colour mode does not use are null.
 struct PrefixClearOut

@@ -135,11 +135,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64
     }
 }
 
-// Small inputs: one workgroup walks the tiles in order with a running carry (one launch).
+// Small inputs: one workgroup walks the tiles in order with a running carry. All SCAN_THREADS
+// threads of the workgroup call it; the outputs are visible to the workgroup after a barrier.
 template <typename T, typename In, typename Out>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
+__device__ __forceinline__ void blockScanAll(In in, Out out, uint64_t n, T* total, T* lds)
 {
-    __shared__ T lds[SCAN_WAVES];
     T carry = 0;
     for (uint64_t tileBase = 0; tileBase < n; tileBase += SCAN_TILE)
     {
@@ -165,6 +165,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint
     }
     if (threadIdx.x == 0 && total)
         *total = carry;
+}
+
+// the same as one launch
+template <typename T, typename In, typename Out>
+__global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
+{
+    __shared__ T lds[SCAN_WAVES];
+    blockScanAll<T>(in, out, n, total, lds);
 }
 
 inline uint64_t scanTiles(uint64_t n)

@@ -207,14 +207,22 @@ void BuildService::run(uint32_t index)
 
         std::string error;
         std::unique_ptr<HostBuildResult> result;
+        double phase[3] = {0, 0, 0};  // setup, generate, capture (ms)
         try
         {
             if (releaseFirst)
                 slot.builder->releaseArena();
+            const auto t0 = Clock::now();
             slot.builder->setup(session->cloudInfo(), session->processingInfo(), session->hostCloud(),
                                 session->deviceCloud());
+            const auto t1 = Clock::now();
             slot.builder->generate();
-            result = HostBuildResult::capture(*slot.builder, itsPinned.get());
+            const auto t2 = Clock::now();
+            result        = HostBuildResult::capture(*slot.builder, itsPinned.get());
+            const auto t3 = Clock::now();
+            phase[0]      = milliseconds(t0, t1);
+            phase[1]      = milliseconds(t1, t2);
+            phase[2]      = milliseconds(t2, t3);
         }
         catch (const std::exception& e)
         {
@@ -236,6 +244,8 @@ void BuildService::run(uint32_t index)
             itsCompleted++;
             itsLatencies.push_back(milliseconds(r.submitted, r.finished));
             itsWaits.push_back(milliseconds(r.submitted, r.started));
+            for (int p = 0; p < 3; ++p)
+                itsPhaseMs[p] += phase[p];
         }
         else
         {
@@ -284,11 +294,13 @@ std::vector<double> BuildService::stats()
         mean /= itsLatencies.size();
         meanWait /= itsWaits.size();
     }
+    const double done = itsCompleted ? static_cast<double>(itsCompleted) : 1.0;
     return {static_cast<double>(itsSubmitted), static_cast<double>(itsCompleted), static_cast<double>(itsFailed),
             static_cast<double>(itsQueue.size()), static_cast<double>(running), static_cast<double>(itsMaxQueue),
             percentile(itsLatencies, 0.50), percentile(itsLatencies, 0.99), mean, static_cast<double>(reserved),
             static_cast<double>(itsBudget), static_cast<double>(itsActive), static_cast<double>(itsSlots.size()),
-            meanWait, maxLatency, static_cast<double>(itsPeakReserved)};
+            meanWait, maxLatency, static_cast<double>(itsPeakReserved), itsPhaseMs[0] / done,
+            itsPhaseMs[1] / done, itsPhaseMs[2] / done};
 }
 
 void BuildService::resetStats()
@@ -299,6 +311,7 @@ void BuildService::resetStats()
     itsPeakReserved                         = 0;
     itsLatencies.clear();
     itsWaits.clear();
+    itsPhaseMs[0] = itsPhaseMs[1] = itsPhaseMs[2] = 0;
 }
 
 }  // namespace oc

@@ -79,6 +79,7 @@ private:
     bool itsStop = false;
 
     uint64_t itsSubmitted = 0, itsCompleted = 0, itsFailed = 0, itsMaxQueue = 0, itsPeakReserved = 0;
+    double itsPhaseMs[3] = {0, 0, 0};  // summed host time of completed requests: setup, generate, capture
     std::vector<double> itsLatencies, itsWaits;
 };
 

@@ -74,7 +74,7 @@ SUMMARY_FIELDS = ("sparse_nodes", "points", "capacity", "used", "internal_nodes"
 TIMING_FIELDS = ("count_ms", "merge_ms", "distribute_ms", "subsample_ms", "build_ms", "insert_ms", "emit_ms")
 SERVICE_STAT_FIELDS = ("submitted", "completed", "failed", "queued", "running", "max_queue_depth", "p50_ms", "p99_ms",
                        "mean_ms", "reserved_bytes", "budget_bytes", "active_slots", "slots", "mean_wait_ms",
-                       "max_ms", "peak_reserved_bytes")
+                       "max_ms", "peak_reserved_bytes", "setup_ms", "generate_ms", "capture_ms")
 
 _api = None
 

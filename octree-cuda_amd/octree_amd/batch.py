@@ -102,7 +102,8 @@ def run_batch(service: api.Service, requests, slots: int | None = None):
     return {"clouds": len(requests), "points": points, "seconds": wall, "points_per_s": points / wall,
             "clouds_per_s": len(requests) / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
             "max_queue_depth": st["max_queue_depth"], "slots": int(st["active_slots"]),
-            "peak_reserved_bytes": st["peak_reserved_bytes"], "failed": int(st["failed"])}
+            "peak_reserved_bytes": st["peak_reserved_bytes"], "failed": int(st["failed"]),
+            "setup_ms": st["setup_ms"], "generate_ms": st["generate_ms"], "capture_ms": st["capture_ms"]}
 
 
 def run_plain_sequential(requests, device: int = 0):
