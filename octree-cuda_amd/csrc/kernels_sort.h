@@ -246,13 +246,22 @@ __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __
     if (threadIdx.x == 0)
         localStart[MAX_BUCKETS] = static_cast<uint16_t>(total);
     __syncthreads();
+    // records into LDS in bucket order; the fine code straight to its final place (this lane
+    // already has the point's Morton code)
+    const uint32_t fineMask = (1u << fineBits) - 1u;
 #pragma unroll
     for (int j = 0; j < STAGE_ITEMS; ++j)
         if (code[j] != ~0u)
-            stage[localStart[code[j] >> fineBits] + rank[j]] = p[j];
+        {
+            const uint32_t b    = code[j] >> fineBits;
+            const uint32_t slot = localStart[b] + rank[j];
+            stage[slot]         = p[j];
+            const uint32_t d    = slot + delta[b];
+            if (d < n)
+                fineCode[d] = static_cast<uint16_t>(code[j] & fineMask);
+        }
     __syncthreads();
 
-    const uint32_t fineMask = (1u << fineBits) - 1u;
     for (uint32_t j = threadIdx.x; j < total; j += SORT_THREADS)
     {
         uint32_t lo = 0, hi = MAX_BUCKETS;  // the last bucket whose first slot is <= j
@@ -267,10 +276,7 @@ __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __
         const uint32_t d = j + delta[lo];
         if (d >= n)
             continue;  // cannot happen: the histogram and this pass compute the same codes
-        const SortedPoint<T> q = stage[j];
-        dst[d]                 = q;
-        fineCode[d]            = static_cast<uint16_t>(
-                pointMorton(static_cast<double>(q.x), static_cast<double>(q.y), static_cast<double>(q.z), g) & fineMask);
+        dst[d] = stage[j];
     }
 }
 
