@@ -33,7 +33,8 @@ void BuildResult::copySnapshot(uint8_t* dst) const
     }
 }
 
-std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& source, PinnedPool* pool)
+std::unique_ptr<HostBuildResult> HostBuildResult::prepare(const BuildResult& source, PinnedPool* pool,
+                                                          uint8_t** block)
 {
     auto r        = std::unique_ptr<HostBuildResult>(new HostBuildResult());
     r->itsCloud   = source.cloud();
@@ -51,9 +52,17 @@ std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& sou
     else
         r->itsBlock = std::shared_ptr<uint8_t>(new uint8_t[bytes], std::default_delete<uint8_t[]>());
     uint8_t* base = r->itsBlock.get();
-    source.copySnapshot(base);
-    r->itsLut = reinterpret_cast<const uint32_t*>(base + l.lut);
-    r->itsOut = base + l.out;
+    r->itsLut     = reinterpret_cast<const uint32_t*>(base + l.lut);
+    r->itsOut     = base + l.out;
+    *block        = base;
+    return r;
+}
+
+std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& source, PinnedPool* pool)
+{
+    uint8_t* block = nullptr;
+    auto r         = prepare(source, pool, &block);
+    source.copySnapshot(block);
     return r;
 }
 

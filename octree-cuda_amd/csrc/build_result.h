@@ -98,6 +98,10 @@ public:
     // Copies the source's nodes and the used part of its LUT and output; with a pool, the LUT
     // and output land in one page-locked buffer from it.
     static std::unique_ptr<HostBuildResult> capture(const BuildResult& source, class PinnedPool* pool = nullptr);
+    // The snapshot's metadata and its (not yet filled) buffer; the caller fills *block with the
+    // source's copySnapshot layout before reading the result.
+    static std::unique_ptr<HostBuildResult> prepare(const BuildResult& source, class PinnedPool* pool,
+                                                    uint8_t** block);
 
     const CloudInfo& cloud() const override { return itsCloud; }
     const ProcessingInfo& processing() const override { return itsProc; }

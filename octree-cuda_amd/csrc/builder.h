@@ -100,6 +100,11 @@ public:
     void copyLut(uint32_t* dst, uint64_t count) const override;
     void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
     void copySnapshot(uint8_t* dst) const override;
+    // The snapshot copies on the builder's copy stream, after the build: the host goes on (the
+    // next setup's upload overlaps them); the next setup / generate makes the build stream wait
+    // for them before it overwrites anything they read. waitSnapshot blocks until they are done.
+    void snapshotAsync(uint8_t* dst);
+    void waitSnapshot();
 
 private:
     template <typename T>
@@ -117,6 +122,7 @@ private:
     void placeLevelByLevel();
 
     void syncStream();
+    void waitForSnapshotOnStream();
     template <typename V>
     V readDevice(const V* devicePtr);
     std::array<uint64_t, 2> readDevicePair(const uint64_t* devicePtr);
@@ -124,6 +130,11 @@ private:
 
     int itsDevice;
     hipStream_t itsStream = nullptr;
+    hipStream_t itsCopyStream         = nullptr;  // snapshot copies
+    hipEvent_t itsBuiltEvent          = nullptr;  // the build's last kernel
+    hipEvent_t itsCopiedEvent         = nullptr;  // the snapshot's last copy
+    hipEvent_t itsUploadedEvent       = nullptr;  // the cloud upload of the current setup
+    bool itsSnapshotPending           = false;    // copies enqueued that the build stream has not waited for
     // Size readbacks: a one-thread kernel publishes {value, value, sequence} into coherent pinned
     // host memory and the host spins on the sequence (no staging copy, no blocking-sync wake-up).
     uint64_t* itsMailbox  = nullptr;

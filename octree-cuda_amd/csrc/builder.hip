@@ -119,6 +119,10 @@ OctreeBuilder::OctreeBuilder(int device) : itsDevice(device)
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));
     OC_HIP_CHECK(hipStreamCreateWithFlags(&itsStream, hipStreamNonBlocking));
+    OC_HIP_CHECK(hipStreamCreateWithFlags(&itsCopyStream, hipStreamNonBlocking));
+    OC_HIP_CHECK(hipEventCreateWithFlags(&itsBuiltEvent, hipEventDisableTiming));
+    OC_HIP_CHECK(hipEventCreateWithFlags(&itsCopiedEvent, hipEventDisableTiming));
+    OC_HIP_CHECK(hipEventCreateWithFlags(&itsUploadedEvent, hipEventDisableTiming));
     for (auto& ev : itsEvents)
         OC_HIP_CHECK(hipEventCreate(&ev));
     OC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&itsMailbox), 3 * sizeof(uint64_t), hipHostMallocCoherent));
@@ -179,6 +183,7 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     {
         itsCloudOwned.ensure(cloudBytes);
         OC_HIP_CHECK(hipMemcpyAsync(itsCloudOwned.get(), hostCloud, cloudBytes, hipMemcpyHostToDevice, itsStream));
+        OC_HIP_CHECK(hipEventRecord(itsUploadedEvent, itsStream));
         itsCloudDevice = itsCloudOwned.as<uint8_t>();
     }
 
@@ -206,8 +211,15 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     itsStOut.ensure(sizeof(OutRecord) * stagedCap);
     itsStPts.ensure(coordBytes * stagedCap);
     itsScalars.ensure(sizeof(uint64_t) * SC_COUNT);
-    OC_HIP_CHECK(hipMemsetAsync(itsOut.get(), 0, sizeof(OutRecord) * itsE, itsStream));
-    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+    // The output starts zeroed (the tail past N + I stays zero, A15). The fill goes on the copy
+    // stream, behind the previous build's snapshot copies that still read the output; the build
+    // waits for that stream before its first write to anything a snapshot reads (generate).
+    OC_HIP_CHECK(hipMemsetAsync(itsOut.get(), 0, sizeof(OutRecord) * itsE, itsCopyStream));
+    OC_HIP_CHECK(hipEventRecord(itsCopiedEvent, itsCopyStream));
+    itsSnapshotPending = true;
+    // the caller's cloud buffer is borrowed until this returns: wait for the upload only
+    if (!deviceCloud)
+        OC_HIP_CHECK(hipEventSynchronize(itsUploadedEvent));
     itsReady = true;
 }
 
@@ -235,6 +247,8 @@ uint64_t OctreeBuilder::arenaBytes() const
 
 void OctreeBuilder::releaseArena()
 {
+    OC_HIP_CHECK(hipStreamSynchronize(itsCopyStream));
+    itsSnapshotPending = false;
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
     for (DeviceBuffer* b : arena())
         b->release();
@@ -272,11 +286,22 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
 
 OctreeBuilder::~OctreeBuilder()
 {
+    if (itsCopyStream)
+    {
+        (void)hipStreamSynchronize(itsCopyStream);
+        (void)hipStreamDestroy(itsCopyStream);
+    }
     if (itsStream)
     {
         (void)hipStreamSynchronize(itsStream);
         (void)hipStreamDestroy(itsStream);
     }
+    if (itsBuiltEvent)
+        (void)hipEventDestroy(itsBuiltEvent);
+    if (itsCopiedEvent)
+        (void)hipEventDestroy(itsCopiedEvent);
+    if (itsUploadedEvent)
+        (void)hipEventDestroy(itsUploadedEvent);
     for (auto& ev : itsEvents)
         if (ev)
             (void)hipEventDestroy(ev);
@@ -387,7 +412,8 @@ void OctreeBuilder::generateTyped()
     itsBatchesRun = 0;
 
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_START], itsStream));
-    countPoints<T>();
+    countPoints<T>();  // writes nothing a snapshot reads, so it overlaps the previous one's copies
+    waitForSnapshotOnStream();
     mergeLevels();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_MERGE_END], itsStream));
     distributePoints<T>();
@@ -1031,15 +1057,38 @@ void OctreeBuilder::copyNodes(HostOctree& o) const
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
 }
 
+void OctreeBuilder::waitForSnapshotOnStream()
+{
+    if (itsSnapshotPending)
+    {
+        OC_HIP_CHECK(hipStreamWaitEvent(itsStream, itsCopiedEvent, 0));
+        itsSnapshotPending = false;
+    }
+}
+
 void OctreeBuilder::copySnapshot(uint8_t* dst) const
+{
+    auto* self = const_cast<OctreeBuilder*>(this);
+    self->snapshotAsync(dst);
+    self->waitSnapshot();
+}
+
+void OctreeBuilder::waitSnapshot()
+{
+    OC_HIP_CHECK(hipEventSynchronize(itsCopiedEvent));
+}
+
+void OctreeBuilder::snapshotAsync(uint8_t* dst)
 {
     if (!itsBuilt)
         throw std::runtime_error("the octree has not been generated");
     const SnapshotLayout l(itsM, itsUsed);
     const uint64_t m = itsM;
+    OC_HIP_CHECK(hipEventRecord(itsBuiltEvent, itsStream));
+    OC_HIP_CHECK(hipStreamWaitEvent(itsCopyStream, itsBuiltEvent, 0));
     auto copy = [&](uint64_t offset, const void* src, uint64_t bytes) {
         if (bytes)
-            OC_HIP_CHECK(hipMemcpyAsync(dst + offset, src, bytes, hipMemcpyDeviceToHost, itsStream));
+            OC_HIP_CHECK(hipMemcpyAsync(dst + offset, src, bytes, hipMemcpyDeviceToHost, itsCopyStream));
     };
     copy(l.dataIdx, itsNodeData.get(), 8 * m);
     copy(l.denseKey, itsNodeKey.get(), 4 * m);
@@ -1050,7 +1099,8 @@ void OctreeBuilder::copySnapshot(uint8_t* dst) const
     copy(l.flags, itsNodeFlags.get(), m);
     copy(l.lut, itsLut.get(), 4 * itsUsed);
     copy(l.out, itsOut.get(), sizeof(OutRecord) * itsUsed);
-    OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+    OC_HIP_CHECK(hipEventRecord(itsCopiedEvent, itsCopyStream));
+    itsSnapshotPending = true;
 }
 
 void OctreeBuilder::copyLut(uint32_t* dst, uint64_t count) const

@@ -142,19 +142,85 @@ void BuildService::setActiveSlots(uint32_t active)
     itsWork.notify_all();
 }
 
+// A slot builds one request while the snapshot copies of the one before it finish: after a
+// build, its copies are enqueued on the builder's copy stream and the slot goes on with the next
+// request (the cloud upload and the count stage overlap the copies; the build waits for them
+// before its merge). The finished request is published after that next build, when its copies
+// are long done, or as soon as the slot has nothing else to do.
 void BuildService::run(uint32_t index)
 {
     Slot& slot = *itsSlots[index];
     (void)hipSetDevice(itsDevice);
+
+    struct Done
+    {
+        Session* session = nullptr;
+        std::unique_ptr<HostBuildResult> result;  // valid once the builder's snapshot copies are done
+        std::string error;
+        double phase[3] = {0, 0, 0};  // setup, generate, capture (ms)
+    } pending;
+
+    // publishes `pending` (call without the lock; the lock is taken for the bookkeeping)
+    auto publish = [&](bool stillBusy) {
+        if (!pending.session)
+            return;
+        if (pending.result)
+        {
+            const auto t = Clock::now();
+            try
+            {
+                slot.builder->waitSnapshot();
+                pending.phase[2] += milliseconds(t, Clock::now());
+                pending.session->adoptResult(std::move(pending.result));
+            }
+            catch (const std::exception& e)
+            {
+                pending.error = e.what();
+            }
+        }
+        std::lock_guard<std::mutex> g(itsMutex);
+        auto& r    = pending.session->request();
+        r.finished = Clock::now();
+        if (pending.error.empty())
+        {
+            r.state = Session::Request::DONE;
+            itsCompleted++;
+            itsLatencies.push_back(milliseconds(r.submitted, r.finished));
+            itsWaits.push_back(milliseconds(r.submitted, r.started));
+            for (int p = 0; p < 3; ++p)
+                itsPhaseMs[p] += pending.phase[p];
+        }
+        else
+        {
+            r.state = Session::Request::FAILED;
+            r.error = pending.error;
+            itsFailed++;
+            log(LOG_ERROR, "service build failed: " + pending.error);
+        }
+        pending = Done{};
+        if (!stillBusy)
+        {
+            slot.busy     = false;
+            slot.reserved = slot.builder->arenaBytes();
+        }
+        itsDone.notify_all();
+        itsWork.notify_all();
+    };
+
     std::unique_lock<std::mutex> lk(itsMutex);
     for (;;)
     {
         if (itsStop)
+        {
+            lk.unlock();
+            publish(false);
             return;
-        if (index >= itsActive && slot.reserved > 0)
+        }
+        if (index >= itsActive && (slot.reserved > 0 || pending.session))
         {
             // an inactive slot keeps no memory
             lk.unlock();
+            publish(false);
             slot.builder->releaseArena();
             lk.lock();
             slot.reserved = 0;
@@ -182,6 +248,7 @@ void BuildService::run(uint32_t index)
             {
                 // the head does not fit while this idle slot holds memory: give it back
                 lk.unlock();
+                publish(false);
                 slot.builder->releaseArena();
                 lk.lock();
                 slot.reserved = 0;
@@ -200,64 +267,81 @@ void BuildService::run(uint32_t index)
         }
         if (!session)
         {
+            if (pending.session)
+            {
+                lk.unlock();
+                publish(false);
+                lk.lock();
+                continue;
+            }
             itsWork.wait(lk);
             continue;
         }
         lk.unlock();
 
-        std::string error;
-        std::unique_ptr<HostBuildResult> result;
-        double phase[3] = {0, 0, 0};  // setup, generate, capture (ms)
+        Done next;
+        next.session = session;
         try
         {
             if (releaseFirst)
+            {
+                publish(true);  // the copies read the arena that is about to be freed
                 slot.builder->releaseArena();
+            }
             const auto t0 = Clock::now();
             slot.builder->setup(session->cloudInfo(), session->processingInfo(), session->hostCloud(),
                                 session->deviceCloud());
-            const auto t1 = Clock::now();
-            slot.builder->generate();
-            const auto t2 = Clock::now();
-            result        = HostBuildResult::capture(*slot.builder, itsPinned.get());
-            const auto t3 = Clock::now();
-            phase[0]      = milliseconds(t0, t1);
-            phase[1]      = milliseconds(t1, t2);
-            phase[2]      = milliseconds(t2, t3);
+            next.phase[0] = milliseconds(t0, Clock::now());
         }
         catch (const std::exception& e)
         {
-            error = e.what();
+            next.error = e.what();
         }
         catch (...)
         {
-            error = "unknown error";
+            next.error = "unknown error";
         }
-        if (result)
-            session->adoptResult(std::move(result));
-
+        if (next.error.empty())
+        {
+            try
+            {
+                const auto t1 = Clock::now();
+                slot.builder->generate();
+                next.phase[1] = milliseconds(t1, Clock::now());
+            }
+            catch (const std::exception& e)
+            {
+                next.error = e.what();
+            }
+            catch (...)
+            {
+                next.error = "unknown error";
+            }
+        }
+        publish(true);  // the previous request: its copies finished before this build's merge
+        if (next.error.empty())
+        {
+            try
+            {
+                const auto t2  = Clock::now();
+                uint8_t* block = nullptr;
+                next.result    = HostBuildResult::prepare(*slot.builder, itsPinned.get(), &block);
+                slot.builder->snapshotAsync(block);
+                next.phase[2] = milliseconds(t2, Clock::now());
+            }
+            catch (const std::exception& e)
+            {
+                next.error  = e.what();
+                next.result = nullptr;
+            }
+            catch (...)
+            {
+                next.error  = "unknown error";
+                next.result = nullptr;
+            }
+        }
+        pending = std::move(next);
         lk.lock();
-        auto& r    = session->request();
-        r.finished = Clock::now();
-        if (error.empty())
-        {
-            r.state = Session::Request::DONE;
-            itsCompleted++;
-            itsLatencies.push_back(milliseconds(r.submitted, r.finished));
-            itsWaits.push_back(milliseconds(r.submitted, r.started));
-            for (int p = 0; p < 3; ++p)
-                itsPhaseMs[p] += phase[p];
-        }
-        else
-        {
-            r.state = Session::Request::FAILED;
-            r.error = error;
-            itsFailed++;
-            log(LOG_ERROR, "service build failed: " + error);
-        }
-        slot.busy     = false;
-        slot.reserved = slot.builder->arenaBytes();
-        itsDone.notify_all();
-        itsWork.notify_all();
     }
 }
 
