@@ -173,6 +173,7 @@ private:
 
     // Morton bucket sort of the cloud
     DeviceBuffer itsBucketed, itsSorted, itsFineCode, itsPerm, itsBucketCount, itsBucketStart, itsBucketCursor;
+    DeviceBuffer itsSplitChunks, itsSplitHist;  // the fine sort's split buckets: chunk list, per-chunk histograms
 
     // output region [0, E) (lut: point index, out: 18-byte record): leaf points at [0, N), internal
     // subsamples at [N, N + I) after the final placement. Staging (st*): subsampled points in

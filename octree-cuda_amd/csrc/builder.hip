@@ -234,7 +234,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
             &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
             &itsCellNode,    &itsCellOf,      &itsCompactOf,     &itsRankOf,       &itsPayload,     &itsScalars,
-            &itsScanScratch, &itsLeafList};
+            &itsScanScratch, &itsLeafList,   &itsSplitChunks,   &itsSplitHist};
 }
 
 uint64_t OctreeBuilder::arenaBytes() const
@@ -274,6 +274,8 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     bytes += 8 * mBound + 77 * mBound + n / 1024;           // node keys / counts, node arrays, leaf chunks
     bytes += 22 * e;                                        // lut + output records
     bytes += (2 * sortedPointBytes(cloud) + 2 + 4) * n;     // bucketed, sorted, fine codes, permutation
+    if (n > FINE_SPLIT)                                     // split buckets of the fine sort
+        bytes += (2 * n / FINE_SPLIT + 1) * (8 + 4ull * MAX_FINE_CELLS);
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
     // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
@@ -491,6 +493,29 @@ void OctreeBuilder::countPoints()
     kFineSort<<<nb, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart, fp,
                                                   itsCount.as<uint32_t>(), itsPerm.as<uint32_t>());
     OC_LAUNCH_CHECK();
+    if (n > FINE_SPLIT)
+    {
+        // buckets above FINE_SPLIT points, a chunk per workgroup (at most 2 n / FINE_SPLIT chunks)
+        const uint64_t bound = 2ull * n / FINE_SPLIT + 1;
+        itsSplitChunks.ensure(sizeof(uint64_t) * bound);
+        itsSplitHist.ensure(sizeof(uint32_t) * MAX_FINE_CELLS * bound);
+        uint32_t* chunkCount = reinterpret_cast<uint32_t*>(itsScalars.as<uint64_t>() + SC_SPLIT);
+        scanExclusive<uint32_t>(nb, SplitChunksIn{bucketStart}, SplitChunksOut{itsSplitChunks.as<uint64_t>()},
+                                itsScanScratch.as<uint32_t>(), chunkCount, itsStream);
+        OC_LAUNCH_CHECK();
+        const unsigned grid = static_cast<unsigned>(bound);
+        kFineChunkHist<<<grid, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart,
+                                                             itsSplitChunks.as<uint64_t>(), chunkCount, fp.lc,
+                                                             itsSplitHist.as<uint32_t>());
+        OC_LAUNCH_CHECK();
+        kFineSplitScan<<<grid, SORT_THREADS, 0, itsStream>>>(bucketStart, itsSplitChunks.as<uint64_t>(), chunkCount,
+                                                             fp, itsCount.as<uint32_t>(), itsSplitHist.as<uint32_t>());
+        OC_LAUNCH_CHECK();
+        kFineChunkPerm<<<grid, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart,
+                                                             itsSplitChunks.as<uint64_t>(), chunkCount, fp.lc,
+                                                             itsSplitHist.as<uint32_t>(), itsPerm.as<uint32_t>());
+        OC_LAUNCH_CHECK();
+    }
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));  // the records move once the leaves are known
 }
 

@@ -21,6 +21,7 @@ enum ScalarSlot : int
     SC_LEAF_TOTAL  = 2,
     SC_SEG_TOTAL   = 3,  // packed (tiles << 32 | inputs)
     SC_TABLE_TOTAL = 4,
+    SC_SPLIT       = 5,  // chunks of the fine sort's split buckets
     SC_LEVEL_START = 8,  // u32[MAX_LEVELS + 1] packed into the following slots
     SC_COUNT       = 16
 };

@@ -288,45 +288,38 @@ struct FineSortParams
     uint32_t levelGrid[MAX_LEVELS];
 };
 
-__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
-                                                           const uint32_t* __restrict__ bucketStart, FineSortParams fp,
-                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
+// Dense counts of bucket b's cells on levels 0..lc (kernelPropagatePointCounts,
+// point_count_propagation.cuh:34-86, as differences of the Morton prefix h over the bucket's
+// fine cells; h null: an empty bucket, all zero).
+__device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& fp, const uint32_t* h,
+                                             uint32_t* __restrict__ count)
 {
-    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
-    __shared__ uint32_t scratch[SORT_WAVES];
-    const uint32_t b     = blockIdx.x;
-    const uint32_t cells = 1u << (3 * fp.lc);
-    const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
     const uint32_t bx = compact3(b >> 2), by = compact3(b >> 1), bz = compact3(b);
-
-    if (lo == hi)
+    for (uint32_t k = 0; k <= fp.lc; ++k)
     {
-        // an empty bucket (most of them in a sparse cloud): zero counts on levels 0..lc, no
-        // histogram and no scan
-        for (uint32_t k = 0; k <= fp.lc; ++k)
+        const uint32_t side   = 1u << (fp.lc - k);
+        const uint32_t n3     = side * side * side;
+        const uint64_t gk     = fp.levelGrid[k];
+        const uint64_t offset = fp.levelOffset[k];
+        for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
         {
-            const uint32_t side   = 1u << (fp.lc - k);
-            const uint32_t n3     = side * side * side;
-            const uint64_t gk     = fp.levelGrid[k];
-            const uint64_t offset = fp.levelOffset[k];
-            for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
+            const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
+            uint32_t v = 0;
+            if (h)
             {
-                const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
-                const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
-                               gz = static_cast<uint64_t>(bz) * side + lz;
-                count[offset + gx + gy * gk + gz * gk * gk] = 0;
+                const uint32_t m = morton3(lx, ly, lz);
+                v                = h[(m + 1) << (3 * k)] - h[m << (3 * k)];
             }
+            const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
+                           gz = static_cast<uint64_t>(bz) * side + lz;
+            count[offset + gx + gy * gk + gz * gk * gk] = v;
         }
-        return;
     }
+}
 
-    for (uint32_t c = threadIdx.x; c <= cells; c += SORT_THREADS)
-        h[c] = 0;
-    __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
-        atomicAdd(h + fineCode[i], 1u);
-    __syncthreads();
-
+// In place: the exclusive prefix of h[0 .. cells), the total in h[cells]. All threads.
+__device__ __forceinline__ void bucketScan(uint32_t* h, uint32_t cells, uint32_t* scratch)
+{
     uint32_t carry = 0;
     for (uint32_t r = 0; r < cells; r += SORT_THREADS)
     {
@@ -341,25 +334,39 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
     if (threadIdx.x == 0)
         h[cells] = carry;
     __syncthreads();
+}
 
-    // dense counts of this bucket's cells on levels 0..lc (kernelPropagatePointCounts,
-    // point_count_propagation.cuh:34-86, as differences of the Morton prefix)
-    for (uint32_t k = 0; k <= fp.lc; ++k)
+// Buckets above FINE_SPLIT points are sorted by several workgroups (kFineChunkHist,
+// kFineSplitScan, kFineChunkPerm): one workgroup's time grows with its bucket, and a dense
+// cluster puts more than a million points in one bucket while the median holds a quarter of that.
+constexpr uint32_t FINE_SPLIT = 196608;
+
+__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
+                                                           const uint32_t* __restrict__ bucketStart, FineSortParams fp,
+                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
+    __shared__ uint32_t scratch[SORT_WAVES];
+    const uint32_t b     = blockIdx.x;
+    const uint32_t cells = 1u << (3 * fp.lc);
+    const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
+    if (hi - lo > FINE_SPLIT)
+        return;  // sorted by the split path
+    if (lo == hi)
     {
-        const uint32_t side   = 1u << (fp.lc - k);
-        const uint32_t n3     = side * side * side;
-        const uint64_t gk     = fp.levelGrid[k];
-        const uint64_t offset = fp.levelOffset[k];
-        for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
-        {
-            const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
-            const uint32_t m  = morton3(lx, ly, lz);
-            const uint32_t v  = h[(m + 1) << (3 * k)] - h[m << (3 * k)];
-            const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
-                           gz = static_cast<uint64_t>(bz) * side + lz;
-            count[offset + gx + gy * gk + gz * gk * gk] = v;
-        }
+        // an empty bucket (most of them in a sparse cloud): zero counts, no histogram, no scan
+        bucketCounts(b, fp, nullptr, count);
+        return;
     }
+
+    for (uint32_t c = threadIdx.x; c <= cells; c += SORT_THREADS)
+        h[c] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
+        atomicAdd(h + fineCode[i], 1u);
+    __syncthreads();
+    bucketScan(h, cells, scratch);
+    bucketCounts(b, fp, h, count);
     __syncthreads();
 
     // the permutation into Morton order: 4-byte scattered stores instead of 32-byte records
@@ -367,6 +374,123 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
     {
         const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
         perm[lo + slot]     = i;
+    }
+}
+
+// chunks of the buckets above FINE_SPLIT points: entry = first chunk of the bucket's points,
+// bucket << 32 | part
+struct SplitChunksIn
+{
+    const uint32_t* bucketStart;
+    __device__ uint32_t operator()(uint64_t b) const
+    {
+        const uint32_t n = bucketStart[b + 1] - bucketStart[b];
+        return n > FINE_SPLIT ? (n + FINE_SPLIT - 1) / FINE_SPLIT : 0u;
+    }
+};
+
+struct SplitChunksOut
+{
+    uint64_t* list;
+    __device__ void operator()(uint64_t b, uint32_t prefix, uint32_t v) const
+    {
+        for (uint32_t k = 0; k < v; ++k)
+            list[prefix + k] = (b << 32) | k;
+    }
+};
+
+// Split path, 1: the histogram of one chunk over the bucket's fine cells, to chunkHist[chunk].
+__global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint16_t* __restrict__ fineCode,
+                                                                const uint32_t* __restrict__ bucketStart,
+                                                                const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ chunkCount, uint32_t lc,
+                                                                uint32_t* __restrict__ chunkHist)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS];
+    if (blockIdx.x >= *chunkCount)
+        return;  // the whole workgroup
+    const uint32_t cells = 1u << (3 * lc);
+    const uint64_t e     = chunks[blockIdx.x];
+    const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
+    const uint32_t lo = bucketStart[b] + part * FINE_SPLIT;
+    const uint32_t hi = min(bucketStart[b + 1], lo + FINE_SPLIT);
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+        h[c] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
+        atomicAdd(h + fineCode[i], 1u);
+    __syncthreads();
+    uint32_t* dst = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+        dst[c] = h[c];
+}
+
+// Split path, 2: per split bucket (the workgroup of its part-0 chunk), the bucket's histogram as
+// the sum of its chunks', its scan and dense counts, and each chunk's starting slot per fine cell
+// (the cell's prefix plus the cell's points in the bucket's earlier chunks), in place of the
+// chunk histograms.
+__global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint32_t* __restrict__ bucketStart,
+                                                                const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ chunkCount,
+                                                                FineSortParams fp, uint32_t* __restrict__ count,
+                                                                uint32_t* __restrict__ chunkHist)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
+    __shared__ uint32_t scratch[SORT_WAVES];
+    if (blockIdx.x >= *chunkCount || static_cast<uint32_t>(chunks[blockIdx.x]) != 0)
+        return;  // the whole workgroup: not the first chunk of a bucket
+    const uint32_t b     = static_cast<uint32_t>(chunks[blockIdx.x] >> 32);
+    const uint32_t parts = (bucketStart[b + 1] - bucketStart[b] + FINE_SPLIT - 1) / FINE_SPLIT;
+    const uint32_t cells = 1u << (3 * fp.lc);
+    uint32_t* first      = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+    {
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < parts; ++k)
+            v += first[static_cast<uint64_t>(k) * MAX_FINE_CELLS + c];
+        h[c] = v;
+    }
+    __syncthreads();
+    bucketScan(h, cells, scratch);
+    bucketCounts(b, fp, h, count);
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+    {
+        uint32_t slot = h[c];
+        for (uint32_t k = 0; k < parts; ++k)
+        {
+            uint32_t* e   = first + static_cast<uint64_t>(k) * MAX_FINE_CELLS + c;
+            const uint32_t v = *e;
+            *e            = slot;
+            slot += v;
+        }
+    }
+}
+
+// Split path, 3: one chunk's part of the permutation from its starting slots.
+__global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* __restrict__ fineCode,
+                                                                const uint32_t* __restrict__ bucketStart,
+                                                                const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ chunkCount, uint32_t lc,
+                                                                const uint32_t* __restrict__ chunkHist,
+                                                                uint32_t* __restrict__ perm)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS];
+    if (blockIdx.x >= *chunkCount)
+        return;  // the whole workgroup
+    const uint32_t cells = 1u << (3 * lc);
+    const uint64_t e     = chunks[blockIdx.x];
+    const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
+    const uint32_t base = bucketStart[b];
+    const uint32_t lo   = base + part * FINE_SPLIT;
+    const uint32_t hi   = min(bucketStart[b + 1], lo + FINE_SPLIT);
+    const uint32_t* src = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+        h[c] = src[c];
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
+    {
+        const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
+        perm[base + slot]   = i;
     }
 }
 

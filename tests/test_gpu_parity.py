@@ -91,3 +91,13 @@ def test_c1_shape_reduced():
     cloud = W.generate(spec)
     _, cubic = W.bounding_box(cloud, spec.points, spec.stride, 0)
     run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, wl.build)
+
+
+def test_split_buckets():
+    """Dense clusters in a large box: two coarse buckets hold ~233K points each, above the fine
+    sort's single-workgroup limit, so their histograms, dense counts and permutation come from
+    the chunked path."""
+    print("case split buckets n=3M G=512 g=128", flush=True)
+    cloud, spec, cubic = synthetic(3_000_000, 0, box=(1000.0, 1000.0, 100.0), clusters=4, seed=5)
+    run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3,
+             _spec(512, 128, 10000, (False, True, False, False)))
