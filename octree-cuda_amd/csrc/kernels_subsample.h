@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "device_common.h"
@@ -89,6 +90,16 @@ struct PointStore
             return quantizePoint(p.x, p.y, p.z, p.rgb, cubic, sx, sy, sz);
         }
         return loadRecord(stOut + (pos - N));
+    }
+    // the point's colour r | g << 8 | b << 16 (a staged point: only the record's colour fields)
+    __device__ __forceinline__ uint32_t rgb(uint32_t pos) const
+    {
+        if (pos < N)
+            return sorted[pos].rgb;
+        uint16_t c[3];
+        __builtin_memcpy(c, reinterpret_cast<const uint8_t*>(stOut + (pos - N)) + offsetof(OutRecord, r), sizeof(c));
+        return static_cast<uint32_t>(c[0] & 0xFF) | (static_cast<uint32_t>(c[1] & 0xFF) << 8) |
+               (static_cast<uint32_t>(c[2] & 0xFF) << 16);
     }
     __device__ __forceinline__ void xyz(uint32_t pos, T& x, T& y, T& z) const
     {
@@ -719,13 +730,11 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
     cellCentre(ng.grid, ix, iy, iz, cx, cy, cz);
     T px, py, pz;
     ps.xyz(in.pos, px, py, pz);
-    const OutRecord r = ps.record(in.pos);
     WeightPayload pl;
     pl.ox   = static_cast<float>(static_cast<double>(px) - cx);
     pl.oy   = static_cast<float>(static_cast<double>(py) - cy);
     pl.oz   = static_cast<float>(static_cast<double>(pz) - cz);
-    pl.rgbx = static_cast<uint32_t>(r.r) | (static_cast<uint32_t>(r.g) << 8) | (static_cast<uint32_t>(r.b) << 16) |
-              ((ix & 3u) << 24);
+    pl.rgbx = ps.rgb(in.pos) | ((ix & 3u) << 24);
     const uint32_t idx  = ps.index(in.pos);
     const uint32_t rank = RANDOM ? fmix32(idx ^ ng.salt) : idx;
     const uint32_t slot = cellStart[compactOf[in.q]] + rankOf[in.q];
