@@ -442,8 +442,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void kLevelSetup(uint32_t levelStart,
                                                             uint32_t* __restrict__ internalTotal,
                                                             uint64_t* __restrict__ segTotal)
 {
-    __shared__ uint32_t lds32[SCAN_WAVES];
-    __shared__ uint64_t lds64[SCAN_WAVES];
+    __shared__ uint32_t lds32[SCAN_ITEMS * SCAN_WAVES];
+    __shared__ uint64_t lds64[SCAN_ITEMS * SCAN_WAVES];
     blockScanAll<uint32_t>(InternalIn{lt.flags, levelStart}, InternalOut{lt.internalList, levelStart}, levelNodes,
                            internalTotal, lds32);
     __syncthreads();

@@ -3,11 +3,16 @@
 // Reduce-then-scan in three launches with no inter-workgroup waiting: (1) each workgroup reduces
 // its tiles of SCAN_TILE items, (2) one workgroup scans the tile sums, (3) each workgroup rescans
 // its tiles with the tile offset and hands (index, exclusive prefix, value) to an output functor.
-// Inside a workgroup the scan is a wave64 shuffle scan (6 steps) plus one LDS step across the
-// four waves. Inputs and outputs are functors so the flags / values are generated and consumed in
-// the same pass as the scan (e.g. occupancy flags straight from the counting grid). Inputs of at
-// most SCAN_SINGLE_TILES tiles take one launch of a single workgroup instead (small builds make
-// many small scans, and every launch is a runtime call).
+// A tile is held striped: thread t holds items k * SCAN_THREADS + t (k < SCAN_ITEMS), so each load
+// and store instruction of a wave covers 64 consecutive items, and a full tile is read with no
+// bounds test, all SCAN_ITEMS loads in flight at once. Its scan is SCAN_ITEMS row scans (row k =
+// the SCAN_THREADS items k * SCAN_THREADS ..): a wave64 shuffle scan per row, the wave totals of
+// all rows through LDS, one barrier. Inputs and outputs are functors so the flags / values are
+// generated and consumed in the same pass as the scan (e.g. occupancy flags straight from the
+// counting grid); an output functor is called once per item, in no particular order, and may
+// write the item's own input (in-place scans). Inputs of at most SCAN_SINGLE_TILES tiles take one
+// launch of a single workgroup instead (small builds make many small scans, and every launch is a
+// runtime call).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -57,18 +62,97 @@ __device__ __forceinline__ T blockExclusiveScan(T v, T& total, T* lds)
     return inc - v + off;
 }
 
+// The SCAN_ITEMS items of this thread in the tile starting at tileBase (zero past n).
+template <typename T, typename In>
+__device__ __forceinline__ void loadTile(In in, uint64_t tileBase, uint64_t n, T (&v)[SCAN_ITEMS])
+{
+    const uint64_t first = tileBase + threadIdx.x;
+    if (tileBase + SCAN_TILE <= n)
+    {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            v[k] = in(first + static_cast<uint64_t>(k) * SCAN_THREADS);
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+        {
+            const uint64_t i = first + static_cast<uint64_t>(k) * SCAN_THREADS;
+            v[k]             = i < n ? in(i) : T(0);
+        }
+    }
+}
+
+// Exclusive prefix within the tile of each item of this thread; returns the tile total.
+// lds holds SCAN_ITEMS * SCAN_WAVES values.
+template <typename T>
+__device__ __forceinline__ T tileExclusiveScan(const T (&v)[SCAN_ITEMS], T (&pre)[SCAN_ITEMS], T* lds)
+{
+    const int w = threadIdx.x >> 6;
+    T inc[SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        inc[k] = waveInclusiveScan(v[k]);
+    if ((threadIdx.x & 63) == 63)
+    {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            lds[k * SCAN_WAVES + w] = inc[k];
+    }
+    __syncthreads();
+    T carry = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+    {
+        T row = 0, before = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_WAVES; ++i)
+        {
+            const T s = lds[k * SCAN_WAVES + i];
+            before += (i < w) ? s : T(0);
+            row += s;
+        }
+        pre[k] = carry + before + (inc[k] - v[k]);
+        carry += row;
+    }
+    __syncthreads();
+    return carry;
+}
+
+template <typename T, typename Out>
+__device__ __forceinline__ void storeTile(Out out, uint64_t tileBase, uint64_t n, const T (&v)[SCAN_ITEMS],
+                                          const T (&pre)[SCAN_ITEMS], T offset)
+{
+    const uint64_t first = tileBase + threadIdx.x;
+    if (tileBase + SCAN_TILE <= n)
+    {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            out(first + static_cast<uint64_t>(k) * SCAN_THREADS, offset + pre[k], v[k]);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+    {
+        const uint64_t i = first + static_cast<uint64_t>(k) * SCAN_THREADS;
+        if (i < n)
+            out(i, offset + pre[k], v[k]);
+    }
+}
+
 template <typename T, typename In>
 __global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* tileSums, uint64_t tiles)
 {
     __shared__ T lds[SCAN_WAVES];
     for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
-        uint64_t base = tile * SCAN_TILE + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
-        T s           = 0;
+        T v[SCAN_ITEMS];
+        loadTile<T>(in, tile * SCAN_TILE, n, v);
+        T s = 0;
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k)
-            if (base + k < n)
-                s += in(base + k);
+            s += v[k];
         T total;
         (void)blockExclusiveScan(s, total, lds);
         if (threadIdx.x == 0)
@@ -79,28 +163,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t tiles, T* total)
 {
-    __shared__ T lds[SCAN_WAVES];
+    __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
     T carry = 0;
     for (uint64_t base = 0; base < tiles; base += SCAN_TILE)
     {
-        uint64_t first = base + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
-        T v[SCAN_ITEMS];
-        T s = 0;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            v[k] = (first + k < tiles) ? tileSums[first + k] : T(0);
-            s += v[k];
-        }
-        T chunk;
-        T off = blockExclusiveScan(s, chunk, lds) + carry;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            if (first + k < tiles)
-                tileSums[first + k] = off;
-            off += v[k];
-        }
+        T v[SCAN_ITEMS], pre[SCAN_ITEMS];
+        loadTile<T>([tileSums](uint64_t i) { return tileSums[i]; }, base, tiles, v);
+        const T chunk = tileExclusiveScan(v, pre, lds);
+        storeTile<T>([tileSums](uint64_t i, T p, T) { tileSums[i] = p; }, base, tiles, v, pre, carry);
         carry += chunk;
     }
     if (threadIdx.x == 0 && total)
@@ -111,56 +181,29 @@ template <typename T, typename In, typename Out>
 __global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* tileOffsets,
                                                           uint64_t tiles)
 {
-    __shared__ T lds[SCAN_WAVES];
+    __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
     for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
-        uint64_t base = tile * SCAN_TILE + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
-        T v[SCAN_ITEMS];
-        T s = 0;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            v[k] = (base + k < n) ? in(base + k) : T(0);
-            s += v[k];
-        }
-        T total;
-        T off = blockExclusiveScan(s, total, lds) + tileOffsets[tile];
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            if (base + k < n)
-                out(base + k, off, v[k]);
-            off += v[k];
-        }
+        T v[SCAN_ITEMS], pre[SCAN_ITEMS];
+        loadTile<T>(in, tile * SCAN_TILE, n, v);
+        (void)tileExclusiveScan(v, pre, lds);
+        storeTile<T>(out, tile * SCAN_TILE, n, v, pre, tileOffsets[tile]);
     }
 }
 
 // Small inputs: one workgroup walks the tiles in order with a running carry. All SCAN_THREADS
-// threads of the workgroup call it; the outputs are visible to the workgroup after a barrier.
+// threads of the workgroup call it with the same lds (SCAN_ITEMS * SCAN_WAVES values); the outputs
+// are visible to the workgroup after a barrier.
 template <typename T, typename In, typename Out>
 __device__ __forceinline__ void blockScanAll(In in, Out out, uint64_t n, T* total, T* lds)
 {
     T carry = 0;
     for (uint64_t tileBase = 0; tileBase < n; tileBase += SCAN_TILE)
     {
-        const uint64_t base = tileBase + static_cast<uint64_t>(threadIdx.x) * SCAN_ITEMS;
-        T v[SCAN_ITEMS];
-        T s = 0;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            v[k] = (base + k < n) ? in(base + k) : T(0);
-            s += v[k];
-        }
-        T chunk;
-        T off = blockExclusiveScan(s, chunk, lds) + carry;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; ++k)
-        {
-            if (base + k < n)
-                out(base + k, off, v[k]);
-            off += v[k];
-        }
+        T v[SCAN_ITEMS], pre[SCAN_ITEMS];
+        loadTile<T>(in, tileBase, n, v);
+        const T chunk = tileExclusiveScan(v, pre, lds);
+        storeTile<T>(out, tileBase, n, v, pre, carry);
         carry += chunk;
     }
     if (threadIdx.x == 0 && total)
@@ -171,7 +214,7 @@ __device__ __forceinline__ void blockScanAll(In in, Out out, uint64_t n, T* tota
 template <typename T, typename In, typename Out>
 __global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
 {
-    __shared__ T lds[SCAN_WAVES];
+    __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
     blockScanAll<T>(in, out, n, total, lds);
 }
 
