@@ -1,8 +1,9 @@
 // Device-wide exclusive prefix sum for the builder (sparse ids, leaf blocks, table slices, slots).
 //
 // Reduce-then-scan in three launches with no inter-workgroup waiting: (1) each workgroup reduces
-// its tiles of SCAN_TILE items, (2) one workgroup scans the tile sums, (3) each workgroup rescans
-// its tiles with the tile offset and hands (index, exclusive prefix, value) to an output functor.
+// its contiguous range of tiles of SCAN_TILE items, (2) one workgroup scans the per-workgroup sums,
+// (3) each workgroup rescans its tiles in order from its offset and hands (index, exclusive prefix,
+// value) to an output functor.
 // A tile is held striped: thread t holds items k * SCAN_THREADS + t (k < SCAN_ITEMS), so each load
 // and store instruction of a wave covers 64 consecutive items, and a full tile is read with no
 // bounds test, all SCAN_ITEMS loads in flight at once. Its scan is SCAN_ITEMS row scans (row k =
@@ -141,25 +142,36 @@ __device__ __forceinline__ void storeTile(Out out, uint64_t tileBase, uint64_t n
     }
 }
 
+// Workgroup b of a grid of G takes the contiguous tiles [b * per, (b + 1) * per), per = ceil(tiles / G).
+__device__ __forceinline__ void tileRange(uint64_t tiles, uint64_t& first, uint64_t& last)
+{
+    const uint64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+    first              = blockIdx.x * per;
+    last               = first + per < tiles ? first + per : tiles;
+}
+
 template <typename T, typename In>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* tileSums, uint64_t tiles)
+__global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* blockSums, uint64_t tiles)
 {
     __shared__ T lds[SCAN_WAVES];
-    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    uint64_t first, last;
+    tileRange(tiles, first, last);
+    T s = 0;
+    for (uint64_t tile = first; tile < last; ++tile)
     {
         T v[SCAN_ITEMS];
         loadTile<T>(in, tile * SCAN_TILE, n, v);
-        T s = 0;
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; ++k)
             s += v[k];
-        T total;
-        (void)blockExclusiveScan(s, total, lds);
-        if (threadIdx.x == 0)
-            tileSums[tile] = total;
     }
+    T total;
+    (void)blockExclusiveScan(s, total, lds);
+    if (threadIdx.x == 0)
+        blockSums[blockIdx.x] = total;
 }
 
+// exclusive scan of the per-workgroup sums, in place
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t tiles, T* total)
 {
@@ -178,16 +190,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t
 }
 
 template <typename T, typename In, typename Out>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* tileOffsets,
+__global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* blockOffsets,
                                                           uint64_t tiles)
 {
     __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
-    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+    uint64_t first, last;
+    tileRange(tiles, first, last);
+    T carry = blockOffsets[blockIdx.x];
+    for (uint64_t tile = first; tile < last; ++tile)
     {
         T v[SCAN_ITEMS], pre[SCAN_ITEMS];
         loadTile<T>(in, tile * SCAN_TILE, n, v);
-        (void)tileExclusiveScan(v, pre, lds);
-        storeTile<T>(out, tile * SCAN_TILE, n, v, pre, tileOffsets[tile]);
+        const T chunk = tileExclusiveScan(v, pre, lds);
+        storeTile<T>(out, tile * SCAN_TILE, n, v, pre, carry);
+        carry += chunk;
     }
 }
 
@@ -239,7 +255,7 @@ void scanExclusive(uint64_t n, In in, Out out, T* tileScratch, T* total, hipStre
     }
     unsigned grid = static_cast<unsigned>(tiles < 8192 ? tiles : 8192);
     kScanReduce<T, In><<<grid, SCAN_THREADS, 0, stream>>>(in, n, tileScratch, tiles);
-    kScanTiles<T><<<1, SCAN_THREADS, 0, stream>>>(tileScratch, tiles, total);
+    kScanTiles<T><<<1, SCAN_THREADS, 0, stream>>>(tileScratch, grid, total);
     kScanDown<T, In, Out><<<grid, SCAN_THREADS, 0, stream>>>(in, out, n, tileScratch, tiles);
 }
 
