@@ -341,6 +341,55 @@ __device__ __forceinline__ void bucketScan(uint32_t* h, uint32_t cells, uint32_t
 // cluster puts more than a million points in one bucket while the median holds a quarter of that.
 constexpr uint32_t FINE_SPLIT = 196608;
 
+// Fine codes per thread and step of the fine-sort loops: all of a step's loads are in flight
+// before its LDS atomics (a load -> atomic -> load chain waits out the memory latency per code).
+constexpr uint32_t FINE_ITEMS = 8;
+
+// LDS histogram of fineCode[lo, hi) into h.
+__device__ __forceinline__ void fineHistogram(const uint16_t* __restrict__ fineCode, uint32_t lo, uint32_t hi,
+                                              uint32_t* h)
+{
+    uint32_t i = lo + threadIdx.x;
+    for (; i + (FINE_ITEMS - 1) * SORT_THREADS < hi; i += FINE_ITEMS * SORT_THREADS)
+    {
+        uint32_t c[FINE_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            c[k] = fineCode[i + k * SORT_THREADS];
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            atomicAdd(h + c[k], 1u);
+    }
+    for (; i < hi; i += SORT_THREADS)
+        atomicAdd(h + fineCode[i], 1u);
+}
+
+// perm[base + slot] = i for every i in [lo, hi), slot taken from the running starts h[code].
+__device__ __forceinline__ void finePermute(const uint16_t* __restrict__ fineCode, uint32_t lo, uint32_t hi,
+                                            uint32_t* h, uint32_t base, uint32_t* __restrict__ perm)
+{
+    uint32_t i = lo + threadIdx.x;
+    for (; i + (FINE_ITEMS - 1) * SORT_THREADS < hi; i += FINE_ITEMS * SORT_THREADS)
+    {
+        uint32_t c[FINE_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            c[k] = fineCode[i + k * SORT_THREADS];
+        uint32_t slot[FINE_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            slot[k] = atomicAdd(h + c[k], 1u);
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            perm[base + slot[k]] = i + k * SORT_THREADS;
+    }
+    for (; i < hi; i += SORT_THREADS)
+    {
+        const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
+        perm[base + slot]   = i;
+    }
+}
+
 __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
                                                            const uint32_t* __restrict__ bucketStart, FineSortParams fp,
                                                            uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
@@ -362,19 +411,14 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
     for (uint32_t c = threadIdx.x; c <= cells; c += SORT_THREADS)
         h[c] = 0;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
-        atomicAdd(h + fineCode[i], 1u);
+    fineHistogram(fineCode, lo, hi, h);
     __syncthreads();
     bucketScan(h, cells, scratch);
     bucketCounts(b, fp, h, count);
     __syncthreads();
 
     // the permutation into Morton order: 4-byte scattered stores instead of 32-byte records
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
-    {
-        const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
-        perm[lo + slot]     = i;
-    }
+    finePermute(fineCode, lo, hi, h, lo, perm);
 }
 
 // chunks of the buckets above FINE_SPLIT points: entry = first chunk of the bucket's points,
@@ -417,8 +461,7 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint16_t* _
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
         h[c] = 0;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
-        atomicAdd(h + fineCode[i], 1u);
+    fineHistogram(fineCode, lo, hi, h);
     __syncthreads();
     uint32_t* dst = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
@@ -487,11 +530,7 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* _
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
         h[c] = src[c];
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_THREADS)
-    {
-        const uint32_t slot = atomicAdd(h + fineCode[i], 1u);
-        perm[base + slot]   = i;
-    }
+    finePermute(fineCode, lo, hi, h, base, perm);
 }
 
 // The sorted cloud: gathered records, written in order. The permutation only moves a record
