@@ -607,6 +607,8 @@ __global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp
 // K5: kernelDistributePoints (point_distributing.cuh:31-76) over the sorted cloud. The leaf of a
 // point is the one of its level-0 cell (cellLeaf); its slot is its offset in the leaf's sorted
 // range, so consecutive threads write consecutive records of one leaf block.
+constexpr int DIST_ITEMS = 4;
+
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void kDistributeSorted(const SortedPoint<T>* __restrict__ sorted, uint32_t n,
                                                             Gridding g, const uint32_t* __restrict__ cellLeaf,
@@ -615,18 +617,58 @@ __global__ __launch_bounds__(BLOCK) void kDistributeSorted(const SortedPoint<T>*
                                                             uint32_t* __restrict__ lut, OutRecord* __restrict__ out,
                                                             double sx, double sy, double sz)
 {
-    for (uint64_t s = globalThread(); s < n; s += globalStride())
-    {
-        const SortedPoint<T> p = sorted[s];
+    // DIST_ITEMS records per thread and step, each level of the load chain (record, its leaf,
+    // the leaf's placement) issued for all of them before the next
+    const uint64_t stride = globalStride();
+    auto cellOf = [&](const SortedPoint<T>& p, uint32_t& cell) {
         uint64_t ix, iy, iz;
         gridCoords(static_cast<double>(p.x), static_cast<double>(p.y), static_cast<double>(p.z), g, ix, iy, iz);
-        if (ix >= g.gridSize || iy >= g.gridSize || iz >= g.gridSize)
-            continue;  // only in-grid points are sorted
-        const uint32_t cell = static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
-        const uint32_t leaf = cellLeaf[cell];
-        const uint64_t pos  = dataIdx[leaf] + (s - sortedStart[leaf]);
-        lut[pos]            = p.idx;
-        storeRecord(out + pos, quantizePoint(p.x, p.y, p.z, p.rgb, g, sx, sy, sz));
+        cell = static_cast<uint32_t>(ix + iy * g.gridSize + iz * g.gridSize * g.gridSize);
+        return ix < g.gridSize && iy < g.gridSize && iz < g.gridSize;  // only in-grid points are sorted
+    };
+    for (uint64_t s0 = globalThread(); s0 < n; s0 += stride * DIST_ITEMS)
+    {
+        if (s0 + (DIST_ITEMS - 1) * stride < n)
+        {
+            SortedPoint<T> p[DIST_ITEMS];
+            uint32_t leaf[DIST_ITEMS];
+            bool in[DIST_ITEMS];
+#pragma unroll
+            for (int k = 0; k < DIST_ITEMS; ++k)
+                p[k] = sorted[s0 + k * stride];
+            // the lookups go to cell 0 for a point outside the grid (none is sorted), so every
+            // load is unconditional and only the stores are predicated
+#pragma unroll
+            for (int k = 0; k < DIST_ITEMS; ++k)
+            {
+                uint32_t cell;
+                in[k]   = cellOf(p[k], cell);
+                leaf[k] = cellLeaf[in[k] ? cell : 0u];
+            }
+            uint64_t pos[DIST_ITEMS];
+#pragma unroll
+            for (int k = 0; k < DIST_ITEMS; ++k)
+                pos[k] = dataIdx[leaf[k]] + (s0 + k * stride - sortedStart[leaf[k]]);
+#pragma unroll
+            for (int k = 0; k < DIST_ITEMS; ++k)
+                if (in[k])
+                {
+                    lut[pos[k]] = p[k].idx;
+                    storeRecord(out + pos[k], quantizePoint(p[k].x, p[k].y, p[k].z, p[k].rgb, g, sx, sy, sz));
+                }
+            continue;
+        }
+        for (uint64_t s = s0; s < n; s += stride)
+        {
+            const SortedPoint<T> p = sorted[s];
+            uint32_t cell;
+            if (!cellOf(p, cell))
+                continue;
+            const uint32_t leaf = cellLeaf[cell];
+            const uint64_t pos  = dataIdx[leaf] + (s - sortedStart[leaf]);
+            lut[pos]            = p.idx;
+            storeRecord(out + pos, quantizePoint(p.x, p.y, p.z, p.rgb, g, sx, sy, sz));
+        }
     }
 }
 
