@@ -87,7 +87,58 @@ def compare(gpu, orc, color_tol=0, check_colors=True):
 WEIGHTED_DIFF_FRACTION = 0.01
 
 
-def weighted_color_check(gpu, orc_handle, tol=1, max_fraction=WEIGHTED_DIFF_FRACTION):
+def record_hashes(lut, out, with_colour):
+    """64-bit hash of every record: (point index, quantised position[, colour]), a boost-style
+    combine of the fields finished by the splitmix64 mixer."""
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        h = lut.astype(np.uint64) * golden
+        for f in ("x", "y", "z") + (("r", "g", "b") if with_colour else ()):
+            v = out[f].astype(np.int64).view(np.uint64)
+            h ^= v + golden + (h << np.uint64(6)) + (h >> np.uint64(2))
+        h ^= h >> np.uint64(30)
+        h *= np.uint64(0xBF58476D1CE4E5B9)
+        h ^= h >> np.uint64(27)
+        h *= np.uint64(0x94D049BB133111EB)
+        h ^= h >> np.uint64(31)
+    return h
+
+
+def node_record_sums(data_idx, point_count, nodes, hashes, base=0):
+    """Per node of `nodes`: the wrapping sum of the hashes of its records, positions
+    [data_idx, data_idx + point_count) (hashes[k] is position base + k). Order-free, so two
+    nodes holding the same records in any order have equal sums."""
+    with np.errstate(over="ignore"):
+        cs = np.zeros(len(hashes) + 1, np.uint64)
+        np.cumsum(hashes, dtype=np.uint64, out=cs[1:])
+        a = data_idx[nodes].astype(np.int64) - base
+        e = a + point_count[nodes].astype(np.int64)
+        return cs[e] - cs[a]
+
+
+def compare_record_sets(gpu, orc, colour_nodes, plain_nodes):
+    """Per-node record sets, compared through node_record_sums: with colours for colour_nodes,
+    without for plain_nodes (node id masks). Returns the number of nodes compared."""
+    summ, nodes, lut, out = gpu
+    used = orc.used
+    checked = 0
+    for mask, with_colour in ((colour_nodes, True), (plain_nodes, False)):
+        ids = np.nonzero(mask)[0]
+        if len(ids) == 0:
+            continue
+        lo = int(orc.data_idx[ids].min())
+        hi = int((orc.data_idx[ids] + orc.point_count[ids]).max())
+        g = node_record_sums(orc.data_idx, orc.point_count, ids, record_hashes(lut[lo:hi], out[lo:hi], with_colour), lo)
+        o = node_record_sums(orc.data_idx, orc.point_count, ids,
+                             record_hashes(orc.lut[lo:hi], orc.out[lo:hi], with_colour), lo)
+        bad = np.nonzero(g != o)[0]
+        assert len(bad) == 0, f"{len(bad)} nodes hold other records, first node {ids[bad[0]]}"
+        checked += len(ids)
+    assert used == summ["used"]
+    return checked
+
+
+def weighted_color_check(gpu, orc_handle, tol=1, max_fraction=WEIGHTED_DIFF_FRACTION, node_ids=None):
     """Distance-weighted colours, level by level: every internal node is recomputed by the oracle
     from the GPU's own child records (point index + colour), and the GPU's node colours must be
     within `tol` per channel of that. Float sums are order dependent (atomics in the reference,
@@ -95,10 +146,11 @@ def weighted_color_check(gpu, orc_handle, tol=1, max_fraction=WEIGHTED_DIFF_FRAC
     oracle, a float exp2 of the float squared distance on the GPU), so the tolerance is per level;
     it does not compound across levels. Besides the bound, at most `max_fraction` of the channels
     may differ at all: a weight that is off by more than a few parts in 10^6 shows up as many
-    channels one apart long before any is two apart. Returns (largest difference, channels that
-    differ, channels compared)."""
+    channels one apart long before any is two apart. node_ids restricts the check to those
+    internal nodes (default: all). Returns (largest difference, channels that differ, channels
+    compared)."""
     summ, nodes, lut, out = gpu
-    internal = np.nonzero(nodes.internal)[0]
+    internal = np.nonzero(nodes.internal)[0] if node_ids is None else np.asarray(node_ids)
     rgb = np.stack([out["r"], out["g"], out["b"]], 1)
     worst = 0
     differing = 0

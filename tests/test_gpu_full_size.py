@@ -1,0 +1,84 @@
+"""GPU builds of the three single-cloud workloads at their full sizes against the CPU oracle (run
+with up to 16 host threads): 5,138,448 x 15 B implicit + inter; 25,836,417 x 15 B explicit +
+intra; 119,701,547 x 27 B f64 implicit + inter + distance-weighted, the benchmark workload.
+
+Bit-exact: node keys, dense counts, flags, parents, children, point counts, data offsets of the
+exported nodes (whose ranges must tile the used output), the used size, and every exported
+node's records as a set, compared through order-free per-node sums of 64-bit record hashes
+(point index, quantised position, colour). The subsampled colours of the distance-weighted
+filter are left out of those sums and checked level by level (+-1 per channel against the
+oracle fed with the GPU's own child records) on a seeded sample of internal nodes. hierarchy.bin
+is a function of the compared node arrays; its writer is compared byte for byte with the
+oracle's in the smaller parity cases (writing the 4.7 GB octree.bin here would only time the
+disk).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle_binding as O
+from octree_amd import workloads as W
+from parity import compare_record_sets, fetch, gpu_build, weighted_color_check
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(os.cpu_count() or 1, 16)
+SAMPLE_NODES = 300  # internal nodes whose weighted colours are recomputed by the oracle
+
+
+def _log(t0, msg):
+    print(f"  [{time.time() - t0:6.1f} s] {msg}", flush=True)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
+def test_full_size_workload(name):
+    t0 = time.time()
+    wl = W.WORKLOADS[name]
+    spec, build = wl.cloud, wl.build
+    print(f"case full size {name}: {wl.name}", flush=True)
+    cloud = W.generate(spec)
+    _, cubic = W.bounding_box(cloud, spec.points, spec.stride, spec.cloud_type)
+    scale = [spec.scale] * 3
+    _log(t0, "cloud generated")
+    weighted = bool(build.weighted and build.inter and not build.intra)
+    orc = O.build(cloud, spec.points, spec.stride, spec.cloud_type, cubic, scale, build, threads=THREADS,
+                  keep=weighted)
+    assert orc.status == 0, f"oracle failed with status {orc.status}"
+    _log(t0, f"oracle built with {THREADS} threads: {len(orc.key):,} nodes, {orc.used:,} records")
+    s = gpu_build(cloud, spec.points, spec.stride, spec.cloud_type, cubic, scale, build)
+    try:
+        g = fetch(s)
+        summ, nodes, lut, out = g
+        _log(t0, "GPU built, results copied")
+
+        assert summ["sparse_nodes"] == len(orc.key), (summ["sparse_nodes"], len(orc.key))
+        for field in ("key", "count", "flags", "parent", "children", "point_count"):
+            np.testing.assert_array_equal(getattr(nodes, field), getattr(orc, field), field)
+        fin = orc.finished
+        np.testing.assert_array_equal(nodes.data_idx[fin], orc.data_idx[fin], "data offsets of exported nodes")
+        assert summ["used"] == orc.used, (summ["used"], orc.used)
+        ids = np.nonzero(fin)[0]
+        order = np.argsort(orc.data_idx[ids], kind="stable")
+        starts = orc.data_idx[ids][order].astype(np.int64)
+        ends = starts + orc.point_count[ids][order].astype(np.int64)
+        assert starts[0] == 0 and ends[-1] == orc.used and (starts[1:] == ends[:-1]).all(), \
+            "exported node ranges do not tile the used output"
+        _log(t0, "hierarchy equal")
+
+        leaves, internal = fin & ~orc.internal, fin & orc.internal
+        if weighted:
+            checked = compare_record_sets(g, orc, leaves, internal)
+            rng = np.random.default_rng(7)
+            inner = np.nonzero(internal)[0]
+            sample = np.sort(rng.choice(inner, size=min(SAMPLE_NODES, len(inner)), replace=False))
+            weighted_color_check(g, orc.handle, tol=1, node_ids=sample)
+        else:
+            checked = compare_record_sets(g, orc, fin, np.zeros_like(fin))
+        _log(t0, f"record sets of {checked:,} nodes equal")
+    finally:
+        s.close()
+        if orc.handle is not None:
+            orc.handle.close()
