@@ -538,12 +538,15 @@ void OctreeBuilder::mergeLevels()
     occOut.nodeKey       = itsNodeKey.as<uint32_t>();
     occOut.nodeCount     = itsNodeCount.as<uint32_t>();
     occOut.count         = count;
-    occOut.levelStart    = levelStart;
-    for (uint32_t l = 0; l <= itsDepth; ++l)
-        occOut.offsets[l] = itsLevelOffset[l];
-    occOut.levels = itsDepth + 1;
     scanExclusive<uint32_t>(itsDense, OccupiedIn{count}, occOut, itsScanScratch.as<uint32_t>(),
                             reinterpret_cast<uint32_t*>(sc + SC_M), itsStream);
+    OC_LAUNCH_CHECK();
+    LevelOffsets lo{};
+    for (uint32_t l = 0; l <= itsDepth; ++l)
+        lo.offset[l] = itsLevelOffset[l];
+    lo.levels = itsDepth + 1;
+    kLevelStarts<<<1, 64, 0, itsStream>>>(itsNodeKey.as<uint32_t>(), reinterpret_cast<uint32_t*>(sc + SC_M), lo,
+                                          levelStart);
     OC_LAUNCH_CHECK();
 
     uint32_t hostStart[MAX_LEVELS + 1] = {};

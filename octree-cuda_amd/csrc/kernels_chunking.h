@@ -74,17 +74,14 @@ struct OccupiedIn
     __device__ uint32_t operator()(uint64_t d) const { return count[d] ? 1u : 0u; }
 };
 
-// Sparse id = exclusive rank of the occupied dense cell (A2); the first id of every level is
-// recorded, and the node's key and dense count are written in the same pass.
+// Sparse id = exclusive rank of the occupied dense cell (A2); the node's key and dense count are
+// written in the same pass.
 struct OccupiedOut
 {
     int32_t* denseToSparse;
     uint32_t* nodeKey;
     uint32_t* nodeCount;
     const uint32_t* count;
-    uint32_t* levelStart;
-    uint32_t offsets[MAX_LEVELS];
-    uint32_t levels;
     __device__ void operator()(uint64_t d, uint32_t prefix, uint32_t v) const
     {
         denseToSparse[d] = v ? static_cast<int32_t>(prefix) : -1;
@@ -93,11 +90,34 @@ struct OccupiedOut
             nodeKey[prefix]   = static_cast<uint32_t>(d);
             nodeCount[prefix] = count[d];
         }
-        for (uint32_t l = 0; l < levels; ++l)
-            if (d == offsets[l])
-                levelStart[l] = prefix;
     }
 };
+
+struct LevelOffsets
+{
+    uint32_t offset[MAX_LEVELS];
+    uint32_t levels;
+};
+
+// First sparse id of every level: ids ascend with the dense index and a level is one dense range,
+// so it is the number of node keys below the level's dense offset (a lower bound over the m keys).
+__global__ void kLevelStarts(const uint32_t* __restrict__ nodeKey, const uint32_t* __restrict__ m,
+                             LevelOffsets lo, uint32_t* __restrict__ levelStart)
+{
+    const uint32_t l = threadIdx.x;
+    if (l >= lo.levels)
+        return;
+    uint32_t a = 0, b = *m;
+    while (a < b)
+    {
+        const uint32_t mid = a + (b - a) / 2;
+        if (nodeKey[mid] < lo.offset[l])
+            a = mid + 1;
+        else
+            b = mid;
+    }
+    levelStart[l] = a;
+}
 
 struct LevelParams
 {

@@ -121,6 +121,8 @@ __device__ __forceinline__ T tileExclusiveScan(const T (&v)[SCAN_ITEMS], T (&pre
     return carry;
 }
 
+// out(i, offset + exclusive prefix, value) for each item i of this thread in the tile (items past n
+// skipped).
 template <typename T, typename Out>
 __device__ __forceinline__ void storeTile(Out out, uint64_t tileBase, uint64_t n, const T (&v)[SCAN_ITEMS],
                                           const T (&pre)[SCAN_ITEMS], T offset)
