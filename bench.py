@@ -111,12 +111,14 @@ def run_batched(args):
     from octree_amd import batch
     log("bench: batched builds: generating 256 clouds")
     reqs = batch.batch_requests()
-    svc = api.Service(args.device, slots=8, memory_budget=200 << 30)
+    slots = 16
+    svc = api.Service(args.device, slots=slots, memory_budget=200 << 30)
     try:
-        # steady state: every slot's arena grown once for this batch (growing an arena frees and
-        # reallocates device memory, which synchronises the whole device)
-        batch.run_batch(svc, reqs, 8)
-        best, tuned = batch.tune_slots(svc, reqs)
+        # steady state: every slot's arena grown for this batch (growing an arena frees and
+        # reallocates device memory, which synchronises the whole device), then one more pass
+        batch.run_batch(svc, reqs, slots)
+        batch.run_batch(svc, reqs, slots)
+        best, tuned = batch.tune_slots(svc, reqs, candidates=(1, 2, 4, 8, 12, 16), sample=128, repeats=3)
         log(f"bench: slot tuner {dict((k, round(v / 1e6, 1)) for k, v in tuned.items())} Mpts/s -> {best} slots")
         seq = batch.run_batch(svc, reqs, 1)
         par = batch.run_batch(svc, reqs, best)
