@@ -181,7 +181,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the octree build of one cloud "
-              f"runs on a single MI355X, see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"runs on a single MI355X; building one cloud across several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     wl = W.WORKLOADS[args.workload]
