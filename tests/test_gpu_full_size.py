@@ -32,23 +32,14 @@ def _log(t0, msg):
     print(f"  [{time.time() - t0:6.1f} s] {msg}", flush=True)
 
 
-@pytest.mark.timeout(900)
-@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
-def test_full_size_workload(name):
-    t0 = time.time()
-    wl = W.WORKLOADS[name]
-    spec, build = wl.cloud, wl.build
-    print(f"case full size {name}: {wl.name}", flush=True)
-    cloud = W.generate(spec)
-    _, cubic = W.bounding_box(cloud, spec.points, spec.stride, spec.cloud_type)
-    scale = [spec.scale] * 3
-    _log(t0, "cloud generated")
+def full_compare(cloud, n, stride, cloud_type, cubic, scale, build, t0):
+    """GPU build against the oracle: node arrays, tiling of the output, record sets per node and,
+    for the distance-weighted filter, sampled per-level colours."""
     weighted = bool(build.weighted and build.inter and not build.intra)
-    orc = O.build(cloud, spec.points, spec.stride, spec.cloud_type, cubic, scale, build, threads=THREADS,
-                  keep=weighted)
+    orc = O.build(cloud, n, stride, cloud_type, cubic, scale, build, threads=THREADS, keep=weighted)
     assert orc.status == 0, f"oracle failed with status {orc.status}"
     _log(t0, f"oracle built with {THREADS} threads: {len(orc.key):,} nodes, {orc.used:,} records")
-    s = gpu_build(cloud, spec.points, spec.stride, spec.cloud_type, cubic, scale, build)
+    s = gpu_build(cloud, n, stride, cloud_type, cubic, scale, build)
     try:
         g = fetch(s)
         summ, nodes, lut, out = g
@@ -78,7 +69,55 @@ def test_full_size_workload(name):
         else:
             checked = compare_record_sets(g, orc, fin, np.zeros_like(fin))
         _log(t0, f"record sets of {checked:,} nodes equal")
+        return summ
     finally:
         s.close()
         if orc.handle is not None:
             orc.handle.close()
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
+def test_full_size_workload(name):
+    t0 = time.time()
+    wl = W.WORKLOADS[name]
+    spec, build = wl.cloud, wl.build
+    print(f"case full size {name}: {wl.name}", flush=True)
+    cloud = W.generate(spec)
+    _, cubic = W.bounding_box(cloud, spec.points, spec.stride, spec.cloud_type)
+    _log(t0, "cloud generated")
+    full_compare(cloud, spec.points, spec.stride, spec.cloud_type, cubic, [spec.scale] * 3, build, t0)
+
+
+MODES = [  # (intra, inter, weighted, random)
+    (False, False, False, False),
+    (True, False, False, False),
+    (False, True, False, False),
+    (False, True, True, False),
+    (False, False, False, True),
+    (True, False, False, True),
+    (False, True, False, True),
+    (False, True, True, True),
+]
+
+
+@pytest.fixture(scope="module")
+def large_cloud():
+    """4 M f64 points in the benchmark cloud's shape: the first levels take the multi-kernel path
+    (device-wide scans, cleared compact arrays, node batches) rather than the one-workgroup setup."""
+    spec = W.CloudSpec(4_000_000, 1, (400.0, 400.0, 40.0), 16, 9, offset=1e6)
+    cloud = W.generate(spec)
+    _, cubic = W.bounding_box(cloud, spec.points, spec.stride, spec.cloud_type)
+    return cloud, spec, cubic
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: "-".join(n for n, f in zip(("intra", "inter", "weighted", "random"), m) if f) or "plain")
+def test_large_cloud_every_mode(large_cloud, mode):
+    t0 = time.time()
+    cloud, spec, cubic = large_cloud
+    intra, inter, weighted, rnd = mode
+    build = W.BuildSpec(512, 10000, 3.0, 128, intra=intra, inter=inter, weighted=weighted, random=rnd, seed=13)
+    print(f"case large cloud 4M f64 G=512 g=128 T=10000 mode={mode}", flush=True)
+    summ = full_compare(cloud, spec.points, spec.stride, spec.cloud_type, cubic, [spec.scale] * 3, build, t0)
+    assert summ["internal_nodes"] > 0
