@@ -101,11 +101,13 @@ MODES = [  # (intra, inter, weighted, random)
 ]
 
 
-@pytest.fixture(scope="module")
-def large_cloud():
-    """4 M f64 points in the benchmark cloud's shape: the first levels take the multi-kernel path
-    (device-wide scans, cleared compact arrays, node batches) rather than the one-workgroup setup."""
-    spec = W.CloudSpec(4_000_000, 1, (400.0, 400.0, 40.0), 16, 9, offset=1e6)
+@pytest.fixture(scope="module", params=[1, 0], ids=["f64", "f32"])
+def large_cloud(request):
+    """4 M points in the benchmark cloud's shape, 27-byte f64 or 15-byte f32 records: the first
+    levels take the multi-kernel path (device-wide scans, cleared compact arrays, node batches)
+    rather than the one-workgroup setup."""
+    ctype = request.param
+    spec = W.CloudSpec(4_000_000, ctype, (400.0, 400.0, 40.0), 16, 9, offset=1e6 if ctype == 1 else 0.0)
     cloud = W.generate(spec)
     _, cubic = W.bounding_box(cloud, spec.points, spec.stride, spec.cloud_type)
     return cloud, spec, cubic
@@ -118,6 +120,6 @@ def test_large_cloud_every_mode(large_cloud, mode):
     cloud, spec, cubic = large_cloud
     intra, inter, weighted, rnd = mode
     build = W.BuildSpec(512, 10000, 3.0, 128, intra=intra, inter=inter, weighted=weighted, random=rnd, seed=13)
-    print(f"case large cloud 4M f64 G=512 g=128 T=10000 mode={mode}", flush=True)
+    print(f"case large cloud 4M type={spec.cloud_type} G=512 g=128 T=10000 mode={mode}", flush=True)
     summ = full_compare(cloud, spec.points, spec.stride, spec.cloud_type, cubic, [spec.scale] * 3, build, t0)
     assert summ["internal_nodes"] > 0
