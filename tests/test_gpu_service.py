@@ -58,7 +58,10 @@ def _same(a, b, weighted):
     assert d <= (1 if weighted else 0), f"colours differ by {d}"
 
 
-def test_service_results_equal_builds_alone():
+@pytest.mark.parametrize("slots,copies", [(3, 1), (16, 8)])
+def test_service_results_equal_builds_alone(slots, copies):
+    """copies x the 6 requests through `slots` slots (16 slots and 48 requests: every slot runs its
+    two-deep pipeline of build and result copies several times)."""
     reqs = _requests()
     alone = []
     for r in reqs:
@@ -67,18 +70,19 @@ def test_service_results_equal_builds_alone():
         s.generate()
         alone.append(fetch(s))
         s.close()
-    print("case service mix of", len(reqs), "requests, 3 slots", flush=True)
-    with api.Service(0, slots=3, memory_budget=64 << 30) as svc:
-        sessions = [_session(r) for r in reqs]
+    mix = [i for _ in range(copies) for i in range(len(reqs))]
+    print("case service mix of", len(mix), "requests,", slots, "slots", flush=True)
+    with api.Service(0, slots=slots, memory_budget=64 << 30) as svc:
+        sessions = [_session(reqs[i]) for i in mix]
         for s in sessions:
             svc.submit(s)
         for s in sessions:
             svc.wait(s)
         st = svc.stats()
-        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["completed"] == len(mix) and st["failed"] == 0
         assert st["p99_ms"] >= st["p50_ms"] > 0
-        for r, s, a in zip(reqs, sessions, alone):
-            _same(fetch(s), a, weighted=r[3].weighted)
+        for i, s in zip(mix, sessions):
+            _same(fetch(s), alone[i], weighted=reqs[i][3].weighted)
             t = s.request_timing()
             assert t["total_ms"] >= t["run_ms"] > 0
             s.close()
