@@ -210,6 +210,8 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     itsStLut.ensure(sizeof(uint32_t) * stagedCap);
     itsStOut.ensure(sizeof(OutRecord) * stagedCap);
     itsStPts.ensure(coordBytes * stagedCap);
+    if (itsColorMode == COLOR_INTER_WEIGHTED)
+        itsStRgb.ensure(sizeof(uint32_t) * stagedCap);
     itsScalars.ensure(sizeof(uint64_t) * SC_COUNT);
     // The output starts zeroed (the tail past N + I stays zero, A15). The fill goes on the copy
     // stream, behind the previous build's snapshot copies that still read the output; the build
@@ -230,7 +232,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
             &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
             &itsPerm,
             &itsBucketCount, &itsBucketStart, &itsBucketCursor,  &itsLut,          &itsOut,         &itsStageNode,
-            &itsStLut,       &itsStOut,       &itsStPts,         &itsInternalList, &itsSegCount,    &itsSegSrc,
+            &itsStLut,       &itsStOut,       &itsStPts,         &itsStRgb,        &itsInternalList, &itsSegCount,    &itsSegSrc,
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
             &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
             &itsCellNode,    &itsCellOf,      &itsCompactOf,     &itsRankOf,       &itsPayload,     &itsScalars,
@@ -277,6 +279,8 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     if (n > FINE_SPLIT)                                     // split buckets of the fine sort
         bytes += (2 * n / FINE_SPLIT + 1) * (8 + 4ull * MAX_FINE_CELLS);
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
+    if (proc.useInterCellAvg && proc.useWeightingFunction && !proc.useIntraCellAvg)
+        bytes += 4 * staged;                                // staged colours of the weighted filter
     // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
     const uint64_t wordsPerNode = (rowsPerNode(proc.subsamplingGrid) * proc.subsamplingGrid + 31) / 32;
@@ -736,6 +740,7 @@ void OctreeBuilder::subsampleLevels()
     ps.stLut  = itsStLut.as<uint32_t>();
     ps.stOut  = itsStOut.as<OutRecord>();
     ps.stPts  = itsStPts.as<T>();
+    ps.stRgb  = weighted ? itsStRgb.as<uint32_t>() : nullptr;
     ps.N      = N;
     ps.cubic  = Gridding{itsCloud.bbMin[0], itsCloud.bbMin[1], itsCloud.bbMin[2], cubicSize, itsProc.chunkingGrid};
     ps.sx     = 1.0 / itsCloud.scale[0];

@@ -20,7 +20,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <cstddef>
 #include <cstdint>
 
 #include "device_common.h"
@@ -75,6 +74,7 @@ struct PointStore
     uint32_t* stLut;
     OutRecord* stOut;
     T* stPts;
+    uint32_t* stRgb;  // distance-weighted filter: staged colours r | g << 8 | b << 16 (8 bits each)
     uint32_t N;
     Gridding cubic;
     double sx, sy, sz;
@@ -91,15 +91,11 @@ struct PointStore
         }
         return loadRecord(stOut + (pos - N));
     }
-    // the point's colour r | g << 8 | b << 16 (a staged point: only the record's colour fields)
+    // the point's colour r | g << 8 | b << 16, the low 8 bits of each channel (distance-weighted
+    // filter only: a staged point's colour comes from stRgb, 4 bytes instead of its 18-byte record)
     __device__ __forceinline__ uint32_t rgb(uint32_t pos) const
     {
-        if (pos < N)
-            return sorted[pos].rgb;
-        uint16_t c[3];
-        __builtin_memcpy(c, reinterpret_cast<const uint8_t*>(stOut + (pos - N)) + offsetof(OutRecord, r), sizeof(c));
-        return static_cast<uint32_t>(c[0] & 0xFF) | (static_cast<uint32_t>(c[1] & 0xFF) << 8) |
-               (static_cast<uint32_t>(c[2] & 0xFF) << 16);
+        return pos < N ? sorted[pos].rgb : stRgb[pos - N];
     }
     __device__ __forceinline__ void xyz(uint32_t pos, T& x, T& y, T& z) const
     {
@@ -951,6 +947,8 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
     const uint32_t st = cursor - ps.N + c;  // staging index
     storeRecord(ps.stOut + st, rec);
     ps.stLut[st] = bestIdx;
+    ps.stRgb[st] = static_cast<uint32_t>(rec.r & 0xFF) | (static_cast<uint32_t>(rec.g & 0xFF) << 8) |
+                   (static_cast<uint32_t>(rec.b & 0xFF) << 16);
     T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
     d[0]         = bx;
     d[1]         = by;
