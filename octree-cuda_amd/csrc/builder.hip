@@ -1029,10 +1029,20 @@ void OctreeBuilder::placeInternalNodes()
         placeLevelByLevel();
     if (staged > 0)
     {
-        kPermute<<<gridFor(staged), BLOCK, 0, itsStream>>>(
-                static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
-                itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(), itsLut.as<uint32_t>(),
-                itsOut.as<OutRecord>());
+        const Gridding cubic{itsCloud.bbMin[0], itsCloud.bbMin[1], itsCloud.bbMin[2],
+                             itsCloud.bbMax[0] - itsCloud.bbMin[0], itsProc.chunkingGrid};
+        const double sx = 1.0 / itsCloud.scale[0], sy = 1.0 / itsCloud.scale[1], sz = 1.0 / itsCloud.scale[2];
+        const uint32_t* rgb = itsColorMode == COLOR_INTER_WEIGHTED ? itsStRgb.as<uint32_t>() : nullptr;
+        if (itsCloud.cloudType == CLOUD_FLOAT_UINT8_T)
+            kPermute<float><<<gridFor(staged), BLOCK, 0, itsStream>>>(
+                    static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
+                    itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
+                    itsStPts.as<float>(), rgb, cubic, sx, sy, sz, itsLut.as<uint32_t>(), itsOut.as<OutRecord>());
+        else
+            kPermute<double><<<gridFor(staged), BLOCK, 0, itsStream>>>(
+                    static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
+                    itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
+                    itsStPts.as<double>(), rgb, cubic, sx, sy, sz, itsLut.as<uint32_t>(), itsOut.as<OutRecord>());
         OC_LAUNCH_CHECK();
     }
 }

@@ -940,15 +940,14 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
         if (q < hi[r])
             weigh(payload[q]);
     }
-    OutRecord rec = quantizePoint(bx, by, bz, 0, ps.cubic, ps.sx, ps.sy, ps.sz);
-    rec.r         = static_cast<uint16_t>(acc[0] / acc[3]);
-    rec.g         = static_cast<uint16_t>(acc[1] / acc[3]);
-    rec.b         = static_cast<uint16_t>(acc[2] / acc[3]);
+    // the colour (averages of 8-bit channels: at most 255); the record itself is quantised from
+    // the staged coordinates by the final placement (kPermute)
+    const uint32_t cr = static_cast<uint16_t>(acc[0] / acc[3]);
+    const uint32_t cg = static_cast<uint16_t>(acc[1] / acc[3]);
+    const uint32_t cb = static_cast<uint16_t>(acc[2] / acc[3]);
     const uint32_t st = cursor - ps.N + c;  // staging index
-    storeRecord(ps.stOut + st, rec);
     ps.stLut[st] = bestIdx;
-    ps.stRgb[st] = static_cast<uint32_t>(rec.r & 0xFF) | (static_cast<uint32_t>(rec.g & 0xFF) << 8) |
-                   (static_cast<uint32_t>(rec.b & 0xFF) << 16);
+    ps.stRgb[st] = (cr & 0xFF) | ((cg & 0xFF) << 8) | ((cb & 0xFF) << 16);
     T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
     d[0]         = bx;
     d[1]         = by;
@@ -1050,11 +1049,14 @@ __global__ __launch_bounds__(PLACE_THREADS) void kPlaceLevels(PlaceParams pp, co
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(BLOCK) void kPermute(uint32_t staged, uint32_t N, const uint32_t* __restrict__ stageNode,
                                                    const uint32_t* __restrict__ stage,
                                                    const uint64_t* __restrict__ dataIdx,
                                                    const uint32_t* __restrict__ stLut,
-                                                   const OutRecord* __restrict__ stOut, uint32_t* __restrict__ lut,
+                                                   const OutRecord* __restrict__ stOut, const T* __restrict__ stPts,
+                                                   const uint32_t* __restrict__ stRgb, Gridding cubic, double sx,
+                                                   double sy, double sz, uint32_t* __restrict__ lut,
                                                    OutRecord* __restrict__ out)
 {
     for (uint64_t s = globalThread(); s < staged; s += globalStride())
@@ -1062,7 +1064,13 @@ __global__ __launch_bounds__(BLOCK) void kPermute(uint32_t staged, uint32_t N, c
         uint32_t id  = stageNode[s];
         uint64_t dst = dataIdx[id] + (static_cast<uint64_t>(N) + s - stage[id]);
         lut[dst]     = stLut[s];
-        storeRecord(out + dst, loadRecord(stOut + s));
+        if (stRgb)  // distance-weighted filter: the record from the staged coordinates and colour
+        {
+            const T* q = stPts + 3 * s;
+            storeRecord(out + dst, quantizePoint(q[0], q[1], q[2], stRgb[s], cubic, sx, sy, sz));
+        }
+        else
+            storeRecord(out + dst, loadRecord(stOut + s));
     }
 }
 
