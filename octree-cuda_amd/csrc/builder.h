@@ -186,7 +186,7 @@ private:
             itsTileList;
     DeviceBuffer itsBitmap, itsPrefix, itsNodeFirst;
     DeviceBuffer itsWin, itsColor, itsCellCnt, itsCellStart, itsCellKey, itsCellNode;
-    DeviceBuffer itsCellOf, itsCompactOf, itsRankOf, itsPayload;
+    DeviceBuffer itsCellOf, itsRankOf, itsPayload;
 
     // small device scalars and scan scratch
     DeviceBuffer itsScalars, itsScanScratch;

@@ -235,7 +235,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
             &itsStLut,       &itsStOut,       &itsStPts,         &itsStRgb,        &itsInternalList, &itsSegCount,    &itsSegSrc,
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
             &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
-            &itsCellNode,    &itsCellOf,      &itsCompactOf,     &itsRankOf,       &itsPayload,     &itsScalars,
+            &itsCellNode,    &itsCellOf,      &itsRankOf,        &itsPayload,      &itsScalars,
             &itsScanScratch, &itsLeafList,   &itsSplitChunks,   &itsSplitHist};
 }
 
@@ -281,11 +281,11 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
     if (proc.useInterCellAvg && proc.useWeightingFunction && !proc.useIntraCellAvg)
         bytes += 4 * staged;                                // staged colours of the weighted filter
-    // per-level work: inputs (cell, compact index, rank, payload) and cells (winner, colour sum,
+    // per-level work: inputs (cell, then its dense index; rank; payload) and cells (winner, colour sum,
     // count, start, key, node), both bounded by E; the occupancy bitmaps and their prefix
     const uint64_t wordsPerNode = (rowsPerNode(proc.subsamplingGrid) * proc.subsamplingGrid + 31) / 32;
     const uint64_t levelNodes   = std::min<uint64_t>(mBound, n / std::max<uint32_t>(1, proc.mergingThreshold) + 8);
-    bytes += (12 + sizeof(WeightPayload) + sizeof(uint64_t)) * e + 32 * staged;
+    bytes += (8 + sizeof(WeightPayload) + sizeof(uint64_t)) * e + 32 * staged;
     bytes += std::min<uint64_t>(kBitmapBudget, 8 * wordsPerNode * levelNodes);
     return bytes;
 }
@@ -658,20 +658,17 @@ static void launchAccumulate(bool random, unsigned blocks, hipStream_t s, uint32
                              uint64_t words, const uint32_t* tileList, const uint32_t* segTile,
                              const uint32_t* segCount, const uint32_t* segSrc, const uint32_t* segInput,
                              const NodeGrid* nodeGrid, const uint32_t* bitmap, const uint32_t* prefix,
-                             const uint32_t* cellOf, const PointStore<T>& ps, uint64_t* win, uint64_t* color,
-                             uint32_t* cellCnt, uint32_t* cellKey, uint32_t* cellNode, uint32_t* compactOf,
-                             uint32_t* rankOf)
+                             uint32_t* cellOf, const PointStore<T>& ps, uint64_t* win, uint64_t* color,
+                             uint32_t* cellCnt, uint32_t* cellKey, uint32_t* cellNode, uint32_t* rankOf)
 {
     if (random)
         kAccumulate<T, MODE, true><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
                                                             segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, ps,
-                                                            win, color, cellCnt, cellKey, cellNode, compactOf,
-                                                            rankOf);
+                                                            win, color, cellCnt, cellKey, cellNode, rankOf);
     else
         kAccumulate<T, MODE, false><<<blocks, TILE, 0, s>>>(tileBase, nodeBase, words, tileList, segTile, segCount,
                                                              segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, ps,
-                                                             win, color, cellCnt, cellKey, cellNode, compactOf,
-                                                             rankOf);
+                                                             win, color, cellCnt, cellKey, cellNode, rankOf);
 }
 
 
@@ -829,7 +826,6 @@ void OctreeBuilder::subsampleLevels()
         itsCellOf.ensure(sizeof(uint32_t) * inputs);
         if (weighted)
         {
-            itsCompactOf.ensure(sizeof(uint32_t) * inputs);
             itsRankOf.ensure(sizeof(uint32_t) * inputs);
             itsPayload.ensure((sizeof(WeightPayload) + sizeof(uint64_t)) * inputs);  // payloads, then winner keys
         }
@@ -929,7 +925,7 @@ void OctreeBuilder::subsampleLevels()
                            itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(), itsNodeGrid.as<NodeGrid>(), bitmap,  \
                            prefix, itsCellOf.as<uint32_t>(), ps, itsWin.as<uint64_t>(), color, cellCnt,               \
                            itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(),                                     \
-                        weighted ? itsCompactOf.as<uint32_t>() : nullptr, weighted ? itsRankOf.as<uint32_t>() : nullptr)
+                        weighted ? itsRankOf.as<uint32_t>() : nullptr)
                 case COLOR_NONE: OC_ACC(COLOR_NONE); break;
                 case COLOR_INTRA: OC_ACC(COLOR_INTRA); break;
                 case COLOR_INTER: OC_ACC(COLOR_INTER); break;
@@ -953,14 +949,14 @@ void OctreeBuilder::subsampleLevels()
                         kScatterPayload<T, true><<<blocks, TILE, 0, itsStream>>>(
                                 tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
-                                itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
+                                itsNodeGrid.as<NodeGrid>(), itsCellKey.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
                                 itsPayload.as<WeightPayload>(), winKey);
                     else
                         kScatterPayload<T, false><<<blocks, TILE, 0, itsStream>>>(
                                 tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
-                                itsNodeGrid.as<NodeGrid>(), itsCompactOf.as<uint32_t>(), itsRankOf.as<uint32_t>(),
+                                itsNodeGrid.as<NodeGrid>(), itsCellKey.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
                                 itsPayload.as<WeightPayload>(), winKey);
                     OC_LAUNCH_CHECK();

@@ -612,11 +612,11 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
                                                      const NodeGrid* __restrict__ nodeGrid,
                                                      const uint32_t* __restrict__ bitmap,
                                                      const uint32_t* __restrict__ prefix,
-                                                     const uint32_t* __restrict__ cellOf, PointStore<T> ps,
+                                                     uint32_t* __restrict__ cellOf, PointStore<T> ps,
                                                      uint64_t* __restrict__ win,
                                                      uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
                                                      uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
-                                                     uint32_t* __restrict__ compactOf, uint32_t* __restrict__ rankOf)
+                                                     uint32_t* __restrict__ rankOf)
 {
     // All lanes take part: runs of lanes in the same cell make one atomic per run.
     InputRef in         = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
@@ -636,15 +636,16 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
     }
     if (MODE == COLOR_INTER_WEIGHTED)
     {
-        // the winner is taken from the cell's payload run in the emit
+        // the winner is taken from the cell's payload run in the emit; the input's cell index is
+        // replaced by the cell's dense index (the payload scatter reads the cell back from cellKey)
         uint32_t base = 0;
         if (valid && run.lane == run.end)
             base = atomicAdd(cellCnt + c, static_cast<uint32_t>(run.end - run.start + 1));
         base = __shfl(base, run.end, 64);
         if (valid)
         {
-            compactOf[in.q] = c;
-            rankOf[in.q]    = base + static_cast<uint32_t>(run.lane - run.start);
+            cellOf[in.q] = c;
+            rankOf[in.q] = base + static_cast<uint32_t>(run.lane - run.start);
         }
         return;
     }
@@ -704,7 +705,7 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
                                                          const uint32_t* __restrict__ segSrc,
                                                          const uint32_t* __restrict__ segInput,
                                                          const NodeGrid* __restrict__ nodeGrid,
-                                                         const uint32_t* __restrict__ compactOf,
+                                                         const uint32_t* __restrict__ cellKey,
                                                          const uint32_t* __restrict__ rankOf,
                                                          const uint32_t* __restrict__ cellStart,
                                                          const uint32_t* __restrict__ cellOf, PointStore<T> ps,
@@ -714,11 +715,12 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
     InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
     if (!in.valid)
         return;
-    const uint32_t cell = cellOf[in.q];
-    if (cell == ~0u)
+    const uint32_t c = cellOf[in.q];  // the dense index kAccumulate left there
+    if (c == ~0u)
         return;
-    const NodeGrid ng = nodeGrid[in.k];
-    const uint32_t gs = ng.grid.gridSize;
+    const uint32_t cell = cellKey[c];
+    const NodeGrid ng   = nodeGrid[in.k];
+    const uint32_t gs   = ng.grid.gridSize;
     uint32_t iy, iz;
     rowCoords(cell / gs, gs, iy, iz);
     const uint32_t ix = cell % gs;
@@ -733,7 +735,7 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
     pl.rgbx = ps.rgb(in.pos) | ((ix & 3u) << 24);
     const uint32_t idx  = ps.index(in.pos);
     const uint32_t rank = RANDOM ? fmix32(idx ^ ng.salt) : idx;
-    const uint32_t slot = cellStart[compactOf[in.q]] + rankOf[in.q];
+    const uint32_t slot = cellStart[c] + rankOf[in.q];
     payload[slot]       = pl;
     winKey[slot]        = (static_cast<uint64_t>(rank) << 32) | in.pos;
 }
