@@ -178,7 +178,7 @@ private:
     // output region [0, E) (lut: point index, out: 18-byte record): leaf points at [0, N), internal
     // subsamples at [N, N + I) after the final placement. Staging (st*): subsampled points in
     // level order, read by the next level, placed at the end.
-    DeviceBuffer itsLut, itsOut, itsStageNode, itsStLut, itsStOut, itsStPts, itsStRgb;
+    DeviceBuffer itsLut, itsOut, itsStLut, itsStOut, itsStPts, itsStRgb;
 
     // per-level subsampling workspace: internal node list, input segments and tiles, per-node
     // occupancy bitmaps with their popcount prefix, per-cell (dense index) state, per-input state

@@ -206,7 +206,6 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     itsBucketCount.ensure(sizeof(uint32_t) * (itsBuckets + 1));
     itsBucketStart.ensure(sizeof(uint32_t) * (itsBuckets + 1));
     itsBucketCursor.ensure(sizeof(uint32_t) * (itsBuckets + 1));
-    itsStageNode.ensure(sizeof(uint32_t) * stagedCap);
     itsStLut.ensure(sizeof(uint32_t) * stagedCap);
     itsStOut.ensure(sizeof(OutRecord) * stagedCap);
     itsStPts.ensure(coordBytes * stagedCap);
@@ -231,7 +230,7 @@ std::vector<DeviceBuffer*> OctreeBuilder::arena()
             &itsNodeData,    &itsNodeParent,  &itsNodeChild,     &itsNodeFlags,    &itsNodeSorted,  &itsNodeAux,
             &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
             &itsPerm,
-            &itsBucketCount, &itsBucketStart, &itsBucketCursor,  &itsLut,          &itsOut,         &itsStageNode,
+            &itsBucketCount, &itsBucketStart, &itsBucketCursor,  &itsLut,          &itsOut,
             &itsStLut,       &itsStOut,       &itsStPts,         &itsStRgb,        &itsInternalList, &itsSegCount,    &itsSegSrc,
             &itsSegTile,     &itsSegInput,    &itsNodeGrid,      &itsTileList,     &itsBitmap,      &itsPrefix,
             &itsNodeFirst,   &itsWin,         &itsColor,         &itsCellCnt,      &itsCellStart,   &itsCellKey,
@@ -278,7 +277,8 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     bytes += (2 * sortedPointBytes(cloud) + 2 + 4) * n;     // bucketed, sorted, fine codes, permutation
     if (n > FINE_SPLIT)                                     // split buckets of the fine sort
         bytes += (2 * n / FINE_SPLIT + 1) * (8 + 4ull * MAX_FINE_CELLS);
-    bytes += (4 + 4 + 18 + coordinateBytes(cloud)) * staged; // staging
+    bytes += (4 + 18 + coordinateBytes(cloud)) * staged;     // staging
+    bytes += 8 * (staged / PLACE_CHUNK + 1);                // placement work list
     if (proc.useInterCellAvg && proc.useWeightingFunction && !proc.useIntraCellAvg)
         bytes += 4 * staged;                                // staged colours of the weighted filter
     // per-level work: inputs (cell, then its dense index; rank; payload) and cells (winner, colour sum,
@@ -972,7 +972,7 @@ void OctreeBuilder::subsampleLevels()
                 kEmitWeighted<T><<<emitBlocks, BLOCK, 0, itsStream>>>(
                         cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
                         prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),
-                        itsPayload.as<WeightPayload>(), winKey, ps, itsStageNode.as<uint32_t>());
+                        itsPayload.as<WeightPayload>(), winKey, ps);
                 OC_LAUNCH_CHECK();
             }
             else
@@ -983,8 +983,7 @@ void OctreeBuilder::subsampleLevels()
     kEmit<T, M><<<gridFor(cells), BLOCK, 0, itsStream>>>(cells, k0, wordsPerNode, static_cast<uint32_t>(cursor),      \
                                                          itsNodeGrid.as<NodeGrid>(), bitmap, prefix,                  \
                                                          itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(),       \
-                                                         itsWin.as<uint64_t>(), itsColor.as<uint64_t>(), ps,          \
-                                                         itsStageNode.as<uint32_t>())
+                                                         itsWin.as<uint64_t>(), itsColor.as<uint64_t>(), ps)
                 case COLOR_NONE: OC_EMIT(COLOR_NONE); break;
                 case COLOR_INTRA: OC_EMIT(COLOR_INTRA); break;
                 default: OC_EMIT(COLOR_INTER); break;
@@ -1025,19 +1024,30 @@ void OctreeBuilder::placeInternalNodes()
         placeLevelByLevel();
     if (staged > 0)
     {
+        // (internal node, chunk) work items; the leaf chunk list is free again after the distribution
+        uint64_t* sc = itsScalars.as<uint64_t>();
+        itsLeafList.ensure(sizeof(uint64_t) * (itsM + staged / PLACE_CHUNK + 1));
+        itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(itsM) + 1));
+        scanExclusive<uint32_t>(itsM, PlacedChunksIn{itsNodeFlags.as<uint8_t>(), itsNodePoints.as<uint32_t>()},
+                                LeafOut{itsLeafList.as<uint64_t>()}, itsScanScratch.as<uint32_t>(),
+                                reinterpret_cast<uint32_t*>(sc + SC_TABLE_TOTAL), itsStream);
+        OC_LAUNCH_CHECK();
+        const uint32_t chunks = static_cast<uint32_t>(readDevice(sc + SC_TABLE_TOTAL));
+        if (chunks == 0)
+            return;
         const Gridding cubic{itsCloud.bbMin[0], itsCloud.bbMin[1], itsCloud.bbMin[2],
                              itsCloud.bbMax[0] - itsCloud.bbMin[0], itsProc.chunkingGrid};
         const double sx = 1.0 / itsCloud.scale[0], sy = 1.0 / itsCloud.scale[1], sz = 1.0 / itsCloud.scale[2];
         const uint32_t* rgb = itsColorMode == COLOR_INTER_WEIGHTED ? itsStRgb.as<uint32_t>() : nullptr;
         if (itsCloud.cloudType == CLOUD_FLOAT_UINT8_T)
-            kPermute<float><<<gridFor(staged), BLOCK, 0, itsStream>>>(
-                    static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
-                    itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
+            kPermute<float><<<chunks, BLOCK, 0, itsStream>>>(
+                    itsLeafList.as<uint64_t>(), N, itsNodeStage.as<uint32_t>(), itsNodeData.as<uint64_t>(),
+                    itsNodePoints.as<uint32_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
                     itsStPts.as<float>(), rgb, cubic, sx, sy, sz, itsLut.as<uint32_t>(), itsOut.as<OutRecord>());
         else
-            kPermute<double><<<gridFor(staged), BLOCK, 0, itsStream>>>(
-                    static_cast<uint32_t>(staged), N, itsStageNode.as<uint32_t>(), itsNodeStage.as<uint32_t>(),
-                    itsNodeData.as<uint64_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
+            kPermute<double><<<chunks, BLOCK, 0, itsStream>>>(
+                    itsLeafList.as<uint64_t>(), N, itsNodeStage.as<uint32_t>(), itsNodeData.as<uint64_t>(),
+                    itsNodePoints.as<uint32_t>(), itsStLut.as<uint32_t>(), itsStOut.as<OutRecord>(),
                     itsStPts.as<double>(), rgb, cubic, sx, sy, sz, itsLut.as<uint32_t>(), itsOut.as<OutRecord>());
         OC_LAUNCH_CHECK();
     }

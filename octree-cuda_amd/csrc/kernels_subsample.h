@@ -755,7 +755,7 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
                                                 const uint32_t* __restrict__ cellKey,
                                                 const uint32_t* __restrict__ cellNode,
                                                 const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
-                                                PointStore<T> ps, uint32_t* __restrict__ stageNode)
+                                                PointStore<T> ps)
 {
     for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
     {
@@ -818,7 +818,6 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
         ps.stLut[st] = ps.index(wpos);
         T* d         = ps.stPts + 3 * static_cast<uint64_t>(st);
         ps.xyz(wpos, d[0], d[1], d[2]);
-        stageNode[st] = ng.id;
     }
 }
 
@@ -845,8 +844,7 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
                                                         const uint32_t* __restrict__ cellNode,
                                                         const uint32_t* __restrict__ cellStart,
                                                         const WeightPayload* __restrict__ payload,
-                                                        const uint64_t* __restrict__ winKey, PointStore<T> ps,
-                                                        uint32_t* __restrict__ stageNode)
+                                                        const uint64_t* __restrict__ winKey, PointStore<T> ps)
 {
     // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
     const uint32_t b     = blockIdx.x;
@@ -954,7 +952,6 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
     d[0]         = bx;
     d[1]         = by;
     d[2]         = bz;
-    stageNode[st] = ng.id;
 }
 
 // DFS post-order placement (A6, kernel_helpers.cuh:55-59): subtree sums of subsample sizes
@@ -1051,28 +1048,51 @@ __global__ __launch_bounds__(PLACE_THREADS) void kPlaceLevels(PlaceParams pp, co
     }
 }
 
+// Placement of the subsampled records: the staged records of an internal node are one contiguous
+// range, and so is its final range [dataIdx, dataIdx + pointCount), so every (node, chunk of
+// PLACE_CHUNK records) work item is a contiguous copy.
+constexpr uint32_t PLACE_CHUNK = 4096;
+
+// chunks of an internal node's placement
+struct PlacedChunksIn
+{
+    const uint8_t* flags;
+    const uint32_t* pointCount;
+    __device__ uint32_t operator()(uint64_t i) const
+    {
+        return (flags[i] & NODE_INTERNAL) ? (pointCount[i] + PLACE_CHUNK - 1) / PLACE_CHUNK : 0u;
+    }
+};
+
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void kPermute(uint32_t staged, uint32_t N, const uint32_t* __restrict__ stageNode,
+__global__ __launch_bounds__(BLOCK) void kPermute(const uint64_t* __restrict__ chunkList, uint32_t N,
                                                    const uint32_t* __restrict__ stage,
                                                    const uint64_t* __restrict__ dataIdx,
+                                                   const uint32_t* __restrict__ pointCount,
                                                    const uint32_t* __restrict__ stLut,
                                                    const OutRecord* __restrict__ stOut, const T* __restrict__ stPts,
                                                    const uint32_t* __restrict__ stRgb, Gridding cubic, double sx,
                                                    double sy, double sz, uint32_t* __restrict__ lut,
                                                    OutRecord* __restrict__ out)
 {
-    for (uint64_t s = globalThread(); s < staged; s += globalStride())
+    const uint64_t entry = chunkList[blockIdx.x];
+    const uint32_t id    = static_cast<uint32_t>(entry);
+    const uint32_t first = static_cast<uint32_t>(entry >> 32) * PLACE_CHUNK;
+    const uint32_t total = pointCount[id];
+    const uint32_t cnt   = total - first < PLACE_CHUNK ? total - first : PLACE_CHUNK;
+    const uint64_t src   = static_cast<uint64_t>(stage[id]) - N + first;  // staging index of the chunk
+    const uint64_t dst   = dataIdx[id] + first;
+    for (uint32_t j = threadIdx.x; j < cnt; j += BLOCK)
     {
-        uint32_t id  = stageNode[s];
-        uint64_t dst = dataIdx[id] + (static_cast<uint64_t>(N) + s - stage[id]);
-        lut[dst]     = stLut[s];
+        const uint64_t s = src + j;
+        lut[dst + j]     = stLut[s];
         if (stRgb)  // distance-weighted filter: the record from the staged coordinates and colour
         {
             const T* q = stPts + 3 * s;
-            storeRecord(out + dst, quantizePoint(q[0], q[1], q[2], stRgb[s], cubic, sx, sy, sz));
+            storeRecord(out + dst + j, quantizePoint(q[0], q[1], q[2], stRgb[s], cubic, sx, sy, sz));
         }
         else
-            storeRecord(out + dst, loadRecord(stOut + s));
+            storeRecord(out + dst + j, loadRecord(stOut + s));
     }
 }
 
