@@ -5,13 +5,14 @@ hierarchy.bin and the per-node point sets bit-exactly; colours are bit-exact exc
 distance-weighted filter (float sums in a different order and a different expf: +-1 per channel).
 """
 import itertools
+import json
 
 import numpy as np
 import pytest
 
 import oracle_binding as O
 from octree_amd import workloads as W
-from parity import run_case, synthetic
+from parity import gpu_build, run_case, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -101,3 +102,32 @@ def test_split_buckets():
     cloud, spec, cubic = synthetic(3_000_000, 0, box=(1000.0, 1000.0, 100.0), clusters=4, seed=5)
     run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3,
              _spec(512, 128, 10000, (False, True, False, False)))
+
+
+def test_statistics_report_matches_oracle(tmp_path):
+    """statistics.json of a GPU build (ocpi_export_json_report; updateNodeStatistics,
+    octree.cu:79-157) against the oracle's node statistics of its own tree, which the reference's
+    fillDummyOctree known answers pin: node and leaf counts, min / max points per leaf and the
+    deepest leaf exactly, the float mean and standard deviation to float rounding."""
+    print("case statistics report 300000 f32 G=256 g=64 T=5000", flush=True)
+    cloud, spec, cubic = synthetic(300_000, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=17)
+    build = _spec(256, 64, 5000, MODES[2])
+    orc = O.build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    assert orc.status == 0
+    want = O.node_statistics(orc.point_count, orc.children, orc.flags, len(orc.key) - 1)
+    s = gpu_build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    try:
+        path = tmp_path / "statistics.json"
+        s.export_json_report(str(path))
+        got = json.loads(path.read_text())
+    finally:
+        s.close()
+    nodes, dist = got["resultNodes"], got["pointDistribution"]
+    assert nodes["leafNodeAmount"] == want["leaves"]
+    assert nodes["internalNodeAmount"] == want["internal"]
+    assert nodes["octreeNodes"] == want["leaves"] + want["internal"]
+    assert nodes["maxLeafDepth"] == want["max_leaf_depth"]
+    assert dist["minPointsPerNode"] == want["min"] and dist["maxPointsPerNode"] == want["max"]
+    assert dist["meanPointsPerLeafNode"] == pytest.approx(want["mean"], rel=1e-5)
+    assert dist["stdevPointsPerLeafNode"] == pytest.approx(want["stdev"], rel=1e-5)
+    assert got["overallNodes"]["sparseOctreeNodes"] == len(orc.key)
