@@ -108,7 +108,8 @@ def test_statistics_report_matches_oracle(tmp_path):
     """statistics.json of a GPU build (ocpi_export_json_report; updateNodeStatistics,
     octree.cu:79-157) against the oracle's node statistics of its own tree, which the reference's
     fillDummyOctree known answers pin: node and leaf counts, min / max points per leaf and the
-    deepest leaf exactly, the float mean and standard deviation to float rounding."""
+    deepest leaf exactly, the float mean and standard deviation to float rounding; and the point
+    distribution histogram (ocpi_export_distribution_histogram) against the leaf counts."""
     print("case statistics report 300000 f32 G=256 g=64 T=5000", flush=True)
     cloud, spec, cubic = synthetic(300_000, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=17)
     build = _spec(256, 64, 5000, MODES[2])
@@ -120,6 +121,13 @@ def test_statistics_report_matches_oracle(tmp_path):
         path = tmp_path / "statistics.json"
         s.export_json_report(str(path))
         got = json.loads(path.read_text())
+        hist = {}
+        for width in (500, 0):  # 0: the reference's automatic width (histogram.cu:39-119)
+            h = tmp_path / f"hist{width}.html"
+            s.export_distribution_histogram(str(h), width)
+            text = h.read_text()
+            data = text[text.index("data:[") + 6:]
+            hist[width] = np.array([int(v) for v in data[:data.index("]")].split(",")], np.int64)
     finally:
         s.close()
     nodes, dist = got["resultNodes"], got["pointDistribution"]
@@ -131,3 +139,8 @@ def test_statistics_report_matches_oracle(tmp_path):
     assert dist["meanPointsPerLeafNode"] == pytest.approx(want["mean"], rel=1e-5)
     assert dist["stdevPointsPerLeafNode"] == pytest.approx(want["stdev"], rel=1e-5)
     assert got["overallNodes"]["sparseOctreeNodes"] == len(orc.key)
+    # the distribution histogram: leaves binned by point count from the smallest leaf up
+    leaf_counts = orc.point_count[orc.finished & ~orc.internal].astype(np.int64)
+    want_hist = np.bincount((leaf_counts - leaf_counts.min()) // 500)
+    np.testing.assert_array_equal(hist[500], want_hist, "histogram with bin width 500")
+    assert hist[0].sum() == want["leaves"]
