@@ -705,7 +705,7 @@ bool OctreeBuilder::orderLeafPoints()
     p.lp.depth     = itsDepth;
     p.lp.threshold = itsProc.mergingThreshold;
     p.g            = itsProc.subsamplingGrid;
-    const uint32_t h    = p.g / 2;
+    const uint32_t h    = p.g > 1 ? p.g / 2 : 1;  // cells per axis of a leaf's octant, as kLeafCellOrder
     const uint64_t keys = rowsPerNode(h) * ((h + 31) / 32);
     while (((keys - 1) >> p.keyShift) >= LEAF_KEYS)
         ++p.keyShift;

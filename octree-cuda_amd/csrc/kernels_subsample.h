@@ -329,7 +329,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
     keyer.inv  = keyer.grid.gridSize / keyer.grid.bbSize;
     // the leaf's octant of the parent grid: the low bit of its coordinates one level down
     const uint32_t ll = pl - 1, lgl = p.lp.grid[ll], llin = nodeKey[leaf] - p.lp.offset[ll];
-    keyer.h           = p.g / 2;
+    keyer.h           = p.g > 1 ? p.g / 2 : 1;  // cells per axis of the leaf's octant (1 for a 1-cell grid)
     keyer.ox          = ((llin % lgl) & 1u) * keyer.h;
     keyer.oy          = (((llin / lgl) % lgl) & 1u) * keyer.h;
     keyer.oz          = ((llin / (lgl * lgl)) & 1u) * keyer.h;

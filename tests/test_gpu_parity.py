@@ -144,3 +144,17 @@ def test_statistics_report_matches_oracle(tmp_path):
     want_hist = np.bincount((leaf_counts - leaf_counts.min()) // 500)
     np.testing.assert_array_equal(hist[500], want_hist, "histogram with bin width 500")
     assert hist[0].sum() == want["leaves"]
+
+
+@pytest.mark.parametrize("G,g,T,n,mode,F", [
+    (64, 1, 2000, 50_000, MODES[2], 3.0),        # one cell per node: every node keeps one point
+    (64, 3, 2000, 50_000, MODES[3], 3.0),        # rows shorter than a bitmap word, pencils half empty
+    (64, 37, 2000, 200_000, MODES[7], 3.0),      # odd: a leaf's octant of its parent grid is not exact
+    (128, 100, 5000, 200_000, MODES[5], 3.0),    # not a power of two
+    (256, 256, 5000, 300_000, MODES[2], 10.0),   # 2 MB occupancy bitmap per node
+], ids=["g1", "g3", "g37", "g100", "g256"])
+def test_uneven_and_large_subsampling_grids(G, g, T, n, mode, F):
+    """Subsampling grids the reference accepts (any size) beyond the powers of two up to 128."""
+    print(f"case subsampling grid G={G} g={g} T={T} n={n} mode={mode}", flush=True)
+    cloud, spec, cubic = synthetic(n, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=3)
+    run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, _spec(G, g, T, mode, F=F))
