@@ -158,3 +158,17 @@ def test_uneven_and_large_subsampling_grids(G, g, T, n, mode, F):
     print(f"case subsampling grid G={G} g={g} T={T} n={n} mode={mode}", flush=True)
     cloud, spec, cubic = synthetic(n, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=3)
     run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, _spec(G, g, T, mode, F=F))
+
+
+def test_level_in_several_node_batches():
+    """A level whose occupancy bitmaps exceed the builder's 6 GiB bitmap budget is subsampled in
+    batches of nodes. At g = 512 a node's bitmap and prefix take 32 MiB, so a batch holds 192
+    nodes; level 2 of this tree has 321 internal nodes and takes two batches, the second starting
+    at a node and a tile other than the level's first."""
+    print("case node batches G=64 g=512 T=400 n=400000", flush=True)
+    cloud, spec, cubic = synthetic(400_000, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=3)
+    build = _spec(64, 512, 400, MODES[6], F=12.0)
+    _, orc = run_case(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build, threads=4)
+    starts = np.cumsum([0] + [(64 >> l) ** 3 for l in range(7)])
+    level = np.searchsorted(starts, orc.key, side="right") - 1
+    assert np.bincount(level[orc.internal]).max() > 192, "no level needs two batches"
