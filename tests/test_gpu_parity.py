@@ -12,7 +12,8 @@ import pytest
 
 import oracle_binding as O
 from octree_amd import workloads as W
-from parity import gpu_build, run_case, synthetic
+from octree_amd import api
+from parity import compare, fetch, gpu_build, run_case, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -172,3 +173,24 @@ def test_level_in_several_node_batches():
     starts = np.cumsum([0] + [(64 >> l) ** 3 for l in range(7)])
     level = np.searchsorted(starts, orc.key, side="right") - 1
     assert np.bincount(level[orc.internal]).max() > 192, "no level needs two batches"
+
+
+def test_points_outside_the_box_fail_the_build():
+    """Points below the cubic box's minimum (the reference turns their negative grid coordinates
+    into indices past its grids; points above the maximum are clamped into the top cells, as
+    there) make ocpi_generate_octree return -1 with a message; the same session then builds the
+    cloud with its real box, equal to the oracle's build."""
+    print("case points outside the box n=20000", flush=True)
+    cloud, spec, cubic = synthetic(20_000, 0, seed=21)
+    side = cubic[3] - cubic[0]
+    half = [cubic[0] + side / 2, cubic[1] + side / 2, cubic[2] + side / 2, cubic[3], cubic[4], cubic[5]]
+    build = _spec(64, 32, 2000, MODES[2])
+    s = api.Session(0)
+    try:
+        with pytest.raises(api.OctreeError, match="outside the cubic bounding box"):
+            gpu_build(cloud, spec.points, spec.stride, 0, half, [spec.scale] * 3, build, sess=s)
+        gpu_build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build, sess=s)
+        orc = O.build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+        compare(fetch(s), orc)
+    finally:
+        s.close()
