@@ -257,3 +257,26 @@ def run_serving(service: api.Service, requests, duration: float, sample_every: f
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "mean_queue_depth": float(np.mean(depth)) if depth else 0.0,
             "max_queue_depth": int(st["max_queue_depth"]), "peak_reserved_bytes": st["peak_reserved_bytes"]}
+
+
+def main(argv=None):
+    """Command line: build the batched workload on the service, e.g. under a kernel trace:
+    `python3 -m octree_amd.batch --slots 8 --repeats 2` (run from octree-cuda_amd/, or with it on
+    PYTHONPATH). The first pass grows the slots' arenas; every pass prints one JSON line."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(prog="python3 -m octree_amd.batch")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--repeats", type=int, default=2, help="passes over the batch (the first warms the arenas)")
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+    reqs = batch_requests()
+    with api.Service(args.device, slots=args.slots, memory_budget=200 << 30) as svc:
+        for rep in range(args.repeats):
+            r = run_batch(svc, reqs, args.slots)
+            print(json.dumps({"pass": rep, **r}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
