@@ -12,7 +12,7 @@ import oracle_binding as O
 from hip_memory import DeviceBuffer
 from octree_amd import api
 from octree_amd import workloads as W
-from parity import synthetic
+from parity import gpu_build, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +37,27 @@ def test_gpu_bounding_box_equals_host_formula(cloud_type, n):
         buf.close()
 
 
+def expected_metadata(hierarchy, side, scale, g, flags):
+    """metadata.json as the reference's exporter writes it (potree_exporter.cu:8-32, 113-173): the
+    points and nodes of the exported hierarchy (22-byte entries, u32 point count at byte 2), the
+    cubic side, spacing = side / g, the scale as set, zero offset, in this key order."""
+    nodes = len(hierarchy) // 22
+    points = sum(int.from_bytes(hierarchy[22 * i + 2:22 * i + 6], "little") for i in range(nodes))
+    return {
+        "version": "2.0", "name": "GpuPotreeConverter", "description": "Generated with GpuPotreeConverter",
+        "points": points, "projection": "", "flags": list(flags),
+        "hierarchy": {"firstChunkSize": 22 * nodes, "stepSize": 100, "depth": 20},
+        "offset": [0, 0, 0], "scale": [scale] * 3, "spacing": side / g,
+        "boundingBox": {"min": [0, 0, 0], "max": [side] * 3}, "encoding": "default",
+        "attributes": [
+            {"name": "position", "description": "", "size": 12, "numElements": 3, "elementSize": 4,
+             "type": "int32", "min": [0, 0, 0], "max": [side] * 3},
+            {"name": "rgb", "description": "", "size": 6, "numElements": 3, "elementSize": 2, "type": "uint16",
+             "min": [0, 0, 0], "max": [65536] * 3},
+        ],
+    }
+
+
 def test_cli_writes_the_oracle_hierarchy(tmp_path):
     print("case cli 120000 points f64 inter", flush=True)
     cloud, spec, cubic = synthetic(120_000, 1, box=(25.0, 25.0, 6.0), offset=1e5)
@@ -52,7 +73,33 @@ def test_cli_writes_the_oracle_hierarchy(tmp_path):
     assert orc.status == 0
     assert (out / "hierarchy.bin").read_bytes() == orc.hierarchy
     meta = json.loads((out / "metadata.json").read_text())
-    assert meta["points"] > 0
+    # the CLI uses the replacement scheme, as the reference's (src/argparser.h:160)
+    assert meta == expected_metadata(orc.hierarchy, cubic[3] - cubic[0], float(np.float32(spec.scale)), 32,
+                                     ("REPLACING", "IMPLICIT-RANDOM-SUBSAMPLING", "INTER-CELL-COLOR-FILTERING"))
+    assert list(meta) == list(expected_metadata(b"", 1.0, 1.0, 1, ())), "key order"
     assert os.path.getsize(out / "octree.bin") == int(np.float32(spec.points) * np.float32(3.0)) * 18
     for name in ("statistics.json", "point_distribution.html", "memory_report.html"):
         assert (out / name).exists(), name
+
+
+@pytest.mark.parametrize("replacement,random,intra,inter,flags", [
+    (False, True, True, False, ("ADDITIVE", "EXPLICIT-RANDOM-SUBSAMPLING", "INTRA-CELL-COLOR-FILTERING")),
+    (True, False, False, False, ("REPLACING", "IMPLICIT-RANDOM-SUBSAMPLING")),
+    (False, False, False, True, ("ADDITIVE", "IMPLICIT-RANDOM-SUBSAMPLING", "INTER-CELL-COLOR-FILTERING")),
+])
+def test_export_potree_metadata(tmp_path, replacement, random, intra, inter, flags):
+    """ocpi_export_potree: hierarchy.bin equals the oracle's and metadata.json the reference
+    exporter's fields for each flag combination."""
+    print(f"case export flags={flags}", flush=True)
+    cloud, spec, cubic = synthetic(40_000, 0, seed=9)
+    build = W.BuildSpec(64, 2000, 3.0, 32, intra=intra, inter=inter, random=random, replacement=replacement, seed=3)
+    orc = O.build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    assert orc.status == 0
+    s = gpu_build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    try:
+        s.export_potree(str(tmp_path))
+    finally:
+        s.close()
+    assert (tmp_path / "hierarchy.bin").read_bytes() == orc.hierarchy
+    meta = json.loads((tmp_path / "metadata.json").read_text())
+    assert meta == expected_metadata(orc.hierarchy, cubic[3] - cubic[0], spec.scale, 32, flags)
