@@ -12,7 +12,7 @@ import oracle_binding as O
 from hip_memory import DeviceBuffer
 from octree_amd import api
 from octree_amd import workloads as W
-from parity import gpu_build, synthetic
+from parity import compare, fetch, gpu_build, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -88,8 +88,8 @@ def test_cli_writes_the_oracle_hierarchy(tmp_path):
     (False, False, False, True, ("ADDITIVE", "IMPLICIT-RANDOM-SUBSAMPLING", "INTER-CELL-COLOR-FILTERING")),
 ])
 def test_export_potree_metadata(tmp_path, replacement, random, intra, inter, flags):
-    """ocpi_export_potree: hierarchy.bin equals the oracle's and metadata.json the reference
-    exporter's fields for each flag combination."""
+    """ocpi_export_potree: hierarchy.bin equals the oracle's, octree.bin holds the built records and
+    a zero tail, and metadata.json has the reference exporter's fields for each flag combination."""
     print(f"case export flags={flags}", flush=True)
     cloud, spec, cubic = synthetic(40_000, 0, seed=9)
     build = W.BuildSpec(64, 2000, 3.0, 32, intra=intra, inter=inter, random=random, replacement=replacement, seed=3)
@@ -98,8 +98,16 @@ def test_export_potree_metadata(tmp_path, replacement, random, intra, inter, fla
     s = gpu_build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
     try:
         s.export_potree(str(tmp_path))
+        g = fetch(s)
     finally:
         s.close()
+    compare(g, orc)  # the build's nodes and per-node record sets equal the oracle's
     assert (tmp_path / "hierarchy.bin").read_bytes() == orc.hierarchy
+    # octree.bin: all E records (potree_exporter.cu:56), the used ones as built, a zero tail (A15)
+    summ, out = g[0], g[3]
+    data = (tmp_path / "octree.bin").read_bytes()
+    assert len(data) == orc.capacity * 18
+    assert data[:summ["used"] * 18] == out.tobytes()
+    assert not any(data[summ["used"] * 18:])
     meta = json.loads((tmp_path / "metadata.json").read_text())
     assert meta == expected_metadata(orc.hierarchy, cubic[3] - cubic[0], spec.scale, 32, flags)
