@@ -181,6 +181,43 @@ def weighted_color_check(gpu, orc_handle, tol=1, max_fraction=WEIGHTED_DIFF_FRAC
     return worst, differing, compared
 
 
+def check_against_oracle(g, orc, weighted, sample_nodes=300, seed=7, max_fraction=WEIGHTED_DIFF_FRACTION):
+    """A fetched GPU result against the oracle at any size: node arrays exactly, the exported
+    node ranges tiling the used output, and every exported node's records as a set (order-free
+    per-node hash sums with colours). For the distance-weighted filter the internal nodes'
+    colours are left out of the sums and checked level by level (+-1) on a seeded sample of at
+    most sample_nodes internal nodes (orc must have been built with keep=True), with at most
+    max_fraction of the sampled channels one apart. Returns (nodes whose record sets were
+    compared, weighted channels that differ, weighted channels compared)."""
+    summ, nodes, lut, out = g
+    assert orc.status == 0, f"oracle failed with status {orc.status}"
+    assert summ["sparse_nodes"] == len(orc.key), (summ["sparse_nodes"], len(orc.key))
+    for field in ("key", "count", "flags", "parent", "children", "point_count"):
+        np.testing.assert_array_equal(getattr(nodes, field), getattr(orc, field), field)
+    fin = orc.finished
+    np.testing.assert_array_equal(nodes.data_idx[fin], orc.data_idx[fin], "data offsets of exported nodes")
+    assert summ["used"] == orc.used, (summ["used"], orc.used)
+    ids = np.nonzero(fin)[0]
+    order = np.argsort(orc.data_idx[ids], kind="stable")
+    starts = orc.data_idx[ids][order].astype(np.int64)
+    ends = starts + orc.point_count[ids][order].astype(np.int64)
+    assert starts[0] == 0 and ends[-1] == orc.used and (starts[1:] == ends[:-1]).all(), \
+        "exported node ranges do not tile the used output"
+    leaves, internal = fin & ~orc.internal, fin & orc.internal
+    differing = compared = 0
+    if weighted:
+        checked = compare_record_sets(g, orc, leaves, internal)
+        inner = np.nonzero(internal)[0]
+        if len(inner):
+            rng = np.random.default_rng(seed)
+            sample = np.sort(rng.choice(inner, size=min(sample_nodes, len(inner)), replace=False))
+            _, differing, compared = weighted_color_check(g, orc.handle, tol=1, max_fraction=max_fraction,
+                                                          node_ids=sample)
+    else:
+        checked = compare_record_sets(g, orc, fin, np.zeros_like(fin))
+    return checked, differing, compared
+
+
 def hierarchy_bytes(s):
     with tempfile.TemporaryDirectory() as d:
         s.export_potree(d)

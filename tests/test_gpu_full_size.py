@@ -20,7 +20,7 @@ import pytest
 
 import oracle_binding as O
 from octree_amd import workloads as W
-from parity import compare_record_sets, fetch, gpu_build, weighted_color_check
+from parity import check_against_oracle, fetch, gpu_build
 
 pytestmark = pytest.mark.gpu
 
@@ -45,29 +45,7 @@ def full_compare(cloud, n, stride, cloud_type, cubic, scale, build, t0):
         summ, nodes, lut, out = g
         _log(t0, "GPU built, results copied")
 
-        assert summ["sparse_nodes"] == len(orc.key), (summ["sparse_nodes"], len(orc.key))
-        for field in ("key", "count", "flags", "parent", "children", "point_count"):
-            np.testing.assert_array_equal(getattr(nodes, field), getattr(orc, field), field)
-        fin = orc.finished
-        np.testing.assert_array_equal(nodes.data_idx[fin], orc.data_idx[fin], "data offsets of exported nodes")
-        assert summ["used"] == orc.used, (summ["used"], orc.used)
-        ids = np.nonzero(fin)[0]
-        order = np.argsort(orc.data_idx[ids], kind="stable")
-        starts = orc.data_idx[ids][order].astype(np.int64)
-        ends = starts + orc.point_count[ids][order].astype(np.int64)
-        assert starts[0] == 0 and ends[-1] == orc.used and (starts[1:] == ends[:-1]).all(), \
-            "exported node ranges do not tile the used output"
-        _log(t0, "hierarchy equal")
-
-        leaves, internal = fin & ~orc.internal, fin & orc.internal
-        if weighted:
-            checked = compare_record_sets(g, orc, leaves, internal)
-            rng = np.random.default_rng(7)
-            inner = np.nonzero(internal)[0]
-            sample = np.sort(rng.choice(inner, size=min(SAMPLE_NODES, len(inner)), replace=False))
-            weighted_color_check(g, orc.handle, tol=1, node_ids=sample)
-        else:
-            checked = compare_record_sets(g, orc, fin, np.zeros_like(fin))
+        checked, _, _ = check_against_oracle(g, orc, weighted, sample_nodes=SAMPLE_NODES)
         _log(t0, f"record sets of {checked:,} nodes equal")
         return summ
     finally:

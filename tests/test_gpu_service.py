@@ -1,5 +1,6 @@
 """The build service (continuous batching) on the GPU: every request's result equals the same
-request built alone, the memory budget is respected, and failures stay with their request.
+request built alone, and both equal the oracle's; the memory budget is respected, and failures
+stay with their request.
 
 Equality: node keys, counts, flags, links, point counts, data offsets, used size and per-node
 point sets and positions exactly; colours exactly, except the distance-weighted filter, whose
@@ -8,9 +9,10 @@ float sums depend on the order points arrive in (+-1 per channel, as against the
 import numpy as np
 import pytest
 
+import oracle_binding as O
 from octree_amd import api
 from octree_amd import workloads as W
-from parity import fetch, node_of_position, records_by_node, synthetic
+from parity import WEIGHTED_DIFF_FRACTION, check_against_oracle, fetch, node_of_position, records_by_node, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -64,12 +66,23 @@ def test_service_results_equal_builds_alone(slots, copies):
     two-deep pipeline of build and result copies several times)."""
     reqs = _requests()
     alone = []
-    for r in reqs:
-        s = _session(r)
+    differing = compared = 0
+    for cloud, spec, cubic, build in reqs:
+        s = _session((cloud, spec, cubic, build))
         s.init()
         s.generate()
         alone.append(fetch(s))
         s.close()
+        weighted = bool(build.weighted and build.inter and not build.intra)
+        orc = O.build(cloud, spec.points, spec.stride, spec.cloud_type, cubic, [spec.scale] * 3, build, keep=weighted)
+        try:
+            _, d, c = check_against_oracle(alone[-1], orc, weighted, sample_nodes=10**9, max_fraction=1.0)
+        finally:
+            if orc.handle is not None:
+                orc.handle.close()
+        differing += d
+        compared += c
+    assert differing <= WEIGHTED_DIFF_FRACTION * max(compared, 1), (differing, compared)
     mix = [i for _ in range(copies) for i in range(len(reqs))]
     print("case service mix of", len(mix), "requests,", slots, "slots", flush=True)
     with api.Service(0, slots=slots, memory_budget=64 << 30) as svc:
