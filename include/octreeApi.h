@@ -180,4 +180,21 @@ EXPORTED int ocpi_compute_bounding_box (
         double* realBB,
         double* cubicBB);
 
+/* ocpi_compute_bounding_box without any device allocation. No counterpart in the reference; it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, results and errors as ocpi_compute_bounding_box; the
+ * partial minima and maxima live in a static 49,200-byte device array (one per GPU, held until the
+ * process ends), which bounds the reduction to 1024 workgroups and so fits any point amount. It
+ * runs on the null stream and returns after the result is copied. Calls on one GPU share that
+ * array: within a process they are serialised by a lock; calls from separate processes on one GPU
+ * each have their own copy of it. */
+EXPORTED int ocpi_compute_bounding_box_noalloc (
+        const uint8_t* devicePointCloud,
+        uint32_t pointAmount,
+        uint32_t dataStride,
+        uint8_t cloudType,
+        int device,
+        double* realBB,
+        double* cubicBB);
+
 #endif

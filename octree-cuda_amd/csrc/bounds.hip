@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -93,20 +94,41 @@ void check(hipError_t e, const char* what)
         throw std::runtime_error(std::string(what) + " failed: " + hipGetErrorString(e));
 }
 
-}  // namespace
-
-void realBoundingBox(const uint8_t* deviceCloud, uint32_t n, uint32_t stride, uint8_t cloudType, int device,
-                     double real[6])
+void checkCloud(const uint8_t* deviceCloud, uint32_t n, uint32_t stride, uint8_t cloudType)
 {
     if (!deviceCloud || n == 0)
         throw std::runtime_error("the cloud is empty");
     if (cloudType > 1 || stride < (cloudType == 0 ? 12u : 24u))
         throw std::runtime_error("invalid cloud type or stride");
+}
+
+uint32_t reductionBlocks(uint32_t n)
+{
+    return static_cast<uint32_t>(std::min<uint64_t>(BB_BLOCKS, (static_cast<uint64_t>(n) + BB_THREADS - 1) / BB_THREADS));
+}
+
+constexpr uint32_t SCRATCH_DOUBLES = 6 * (BB_BLOCKS + 1);
+constexpr int MAX_DEVICES = 64;
+std::mutex gScratchMutex[MAX_DEVICES];
+double* gScratchPtr[MAX_DEVICES] = {};
+
+}  // namespace
+
+// The partial minima and maxima of the allocation-free variant: one entry of 6 doubles per
+// workgroup of the first pass (at most BB_BLOCKS) plus the final one, 49,200 bytes per device.
+// In host code this name only stands for the array; its device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+void realBoundingBox(const uint8_t* deviceCloud, uint32_t n, uint32_t stride, uint8_t cloudType, int device,
+                     double real[6])
+{
+    checkCloud(deviceCloud, n, stride, cloudType);
     check(hipSetDevice(device), "hipSetDevice");
     hipStream_t stream;
     check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     double* partials = nullptr;
-    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(BB_BLOCKS, (n + BB_THREADS - 1) / BB_THREADS));
+    const uint32_t blocks = reductionBlocks(n);
     hipError_t e          = hipMalloc(&partials, sizeof(double) * 6 * (blocks + 1));
     if (e == hipSuccess)
     {
@@ -124,6 +146,36 @@ void realBoundingBox(const uint8_t* deviceCloud, uint32_t n, uint32_t stride, ui
     }
     (void)hipStreamDestroy(stream);
     check(e, "bounding box reduction");
+}
+
+void realBoundingBoxNoAlloc(const uint8_t* deviceCloud, uint32_t n, uint32_t stride, uint8_t cloudType, int device,
+                            double real[6])
+{
+    checkCloud(deviceCloud, n, stride, cloudType);
+    if (device < 0 || device >= MAX_DEVICES)
+        throw std::runtime_error("device " + std::to_string(device) + " is out of range");
+    const uint32_t blocks = reductionBlocks(n);
+    if (6ull * (blocks + 1) > SCRATCH_DOUBLES)  // the workgroup cap keeps every cloud size within the array
+        throw std::runtime_error("the reduction needs " + std::to_string(blocks) + " partials, more than the " +
+                                 std::to_string(BB_BLOCKS) + " the static scratch holds");
+    // one call per device at a time: every call on a device shares its scratch array
+    std::lock_guard<std::mutex> lock(gScratchMutex[device]);
+    check(hipSetDevice(device), "hipSetDevice");
+    if (!gScratchPtr[device])
+    {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress of the bounding-box scratch");
+        gScratchPtr[device] = static_cast<double*>(p);
+    }
+    double* partials = gScratchPtr[device];
+    // the null stream: creating a stream would allocate runtime resources
+    if (cloudType == CLOUD_FLOAT_UINT8_T)
+        kBoundsPartial<float><<<blocks, BB_THREADS, 0, nullptr>>>(deviceCloud, n, stride, partials);
+    else
+        kBoundsPartial<double><<<blocks, BB_THREADS, 0, nullptr>>>(deviceCloud, n, stride, partials);
+    kBoundsFinal<<<1, BB_THREADS, 0, nullptr>>>(partials, blocks, partials + 6 * blocks);
+    check(hipGetLastError(), "bounding box reduction launch");
+    check(hipMemcpy(real, partials + 6 * blocks, sizeof(double) * 6, hipMemcpyDeviceToHost), "bounding box copy");
 }
 
 void cubicBoundingBox(const double real[6], double cubic[6])  // src/boundingbox.h:35-52

@@ -343,3 +343,14 @@ int ocpi_compute_bounding_box(const uint8_t* devicePointCloud, uint32_t pointAmo
         oc::cubicBoundingBox(realBB, cubicBB);
     });
 }
+
+int ocpi_compute_bounding_box_noalloc(const uint8_t* devicePointCloud, uint32_t pointAmount, uint32_t dataStride,
+                                      uint8_t cloudType, int device, double* realBB, double* cubicBB)
+{
+    return execute([&] {
+        if (!realBB || !cubicBB)
+            throw std::runtime_error("bounding box output pointers are null");
+        oc::realBoundingBoxNoAlloc(devicePointCloud, pointAmount, dataStride, cloudType, device, realBB);
+        oc::cubicBoundingBox(realBB, cubicBB);
+    });
+}

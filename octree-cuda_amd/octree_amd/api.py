@@ -58,6 +58,7 @@ SIGNATURES = {
     "ocpi_get_request_timing": [_vp, _vp, _u32],
     "ocpi_estimate_footprint": [_vp, _vp],
     "ocpi_compute_bounding_box": [_vp, _u32, _u32, _u8, ctypes.c_int, _vp, _vp],
+    "ocpi_compute_bounding_box_noalloc": [_vp, _u32, _u32, _u8, ctypes.c_int, _vp, _vp],
 }
 
 SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
@@ -67,7 +68,7 @@ SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_ser
 REFERENCE_ENTRY_POINTS = [n for n in SIGNATURES if n not in [
     "ocpi_set_point_cloud_device", "ocpi_set_random_seed", "ocpi_get_last_error", "ocpi_get_build_summary",
     "ocpi_copy_nodes", "ocpi_copy_point_lut", "ocpi_copy_output", "ocpi_get_stage_timings",
-    "ocpi_compute_bounding_box"] + SERVICE_ENTRY_POINTS]
+    "ocpi_compute_bounding_box", "ocpi_compute_bounding_box_noalloc"] + SERVICE_ENTRY_POINTS]
 
 SUMMARY_FIELDS = ("sparse_nodes", "points", "capacity", "used", "internal_nodes", "leaf_nodes", "depth",
                   "dense_nodes", "exported_nodes")
