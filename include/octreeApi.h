@@ -180,6 +180,31 @@ EXPORTED int ocpi_compute_bounding_box (
         double* realBB,
         double* cubicBB);
 
+/* ---- batched builds: many clouds, one launch per stage ---------------------------------------
+ * No counterpart in the reference, which builds one cloud per process and launches every kernel
+ * per phase and per node (kernel_executor.cuh:8-28). A batch builder takes configured sessions
+ * (setters and configure_* as for ocpi_init_octree; host or device clouds) and builds them
+ * together: the sessions that share record type, chunking grid, colour filter and subsampling
+ * method go through every stage in one launch per stage, with per-cloud offsets into a shared
+ * device arena. Each session's result is attached as after ocpi_generate_octree (node arrays,
+ * data offsets and record sets equal a build of that session alone). The builder's arena stays
+ * resident between batches. */
+EXPORTED int ocpi_create_batch_builder (void** builder, int device);
+EXPORTED int ocpi_destroy_batch_builder (void* builder);
+/* Builds sessions[0 .. count). status (may be NULL) receives 0 or -1 per session; the call returns
+ * -1 when any session failed (the first failure is the last error), and the others keep their
+ * results. The sessions' host clouds must stay valid until the call returns. */
+EXPORTED int ocpi_batch_build (void* builder, void** sessions, uint32_t count, int32_t* status);
+/* Persistent workgroups per launch of the batched kernels: those with one item per thread, one
+ * subsampling tile per workgroup and one sort chunk per workgroup (each 1..65536). */
+EXPORTED int ocpi_batch_set_launch (void* builder, uint32_t threadGrid, uint32_t tileGrid, uint32_t sortGrid);
+/* Of the last ocpi_batch_build, summed over its groups: device ms of [0] counting [1] merge
+ * [2] distribution [3] subsampling and placement [4] the whole pipeline; host ms of [5] the whole
+ * call [6] the result copies. */
+EXPORTED int ocpi_batch_get_timings (void* builder, float* milliseconds, uint32_t n);
+/* Device bytes a batch of these sessions needs (validates them). */
+EXPORTED int ocpi_estimate_batch_footprint (void** sessions, uint32_t count, uint64_t* bytes);
+
 /* ocpi_compute_bounding_box without any device allocation. No counterpart in the reference; it is
  * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc). Same arguments, results and errors as ocpi_compute_bounding_box; the

@@ -52,9 +52,33 @@ std::unique_ptr<HostBuildResult> HostBuildResult::prepare(const BuildResult& sou
     else
         r->itsBlock = std::shared_ptr<uint8_t>(new uint8_t[bytes], std::default_delete<uint8_t[]>());
     uint8_t* base = r->itsBlock.get();
+    r->itsBase    = base;
     r->itsLut     = reinterpret_cast<const uint32_t*>(base + l.lut);
     r->itsOut     = base + l.out;
     *block        = base;
+    return r;
+}
+
+std::unique_ptr<HostBuildResult> HostBuildResult::inBlock(const CloudInfo& cloud, const ProcessingInfo& proc,
+                                                          uint32_t depth, uint32_t denseNodes, uint32_t sparseNodes,
+                                                          uint64_t capacity, uint64_t used,
+                                                          const std::vector<float>& timings,
+                                                          std::shared_ptr<uint8_t> block, uint8_t* base)
+{
+    auto r        = std::unique_ptr<HostBuildResult>(new HostBuildResult());
+    r->itsCloud   = cloud;
+    r->itsProc    = proc;
+    r->itsDepth   = depth;
+    r->itsDense   = denseNodes;
+    r->itsM       = sparseNodes;
+    r->itsE       = capacity;
+    r->itsUsed    = used;
+    r->itsTimings = timings;
+    r->itsBlock   = std::move(block);
+    const SnapshotLayout l(sparseNodes, used);
+    r->itsBase = base;
+    r->itsLut  = reinterpret_cast<const uint32_t*>(base + l.lut);
+    r->itsOut  = base + l.out;
     return r;
 }
 
@@ -69,7 +93,7 @@ std::unique_ptr<HostBuildResult> HostBuildResult::capture(const BuildResult& sou
 void HostBuildResult::copyNodes(HostOctree& o) const
 {
     const SnapshotLayout l(itsM, itsUsed);
-    const uint8_t* base = itsBlock.get();
+    const uint8_t* base = itsBase;
     const uint64_t m    = itsM;
     auto take = [&](auto& v, uint64_t offset, uint64_t n) {
         using V = typename std::decay_t<decltype(v)>::value_type;

@@ -102,6 +102,13 @@ public:
     // source's copySnapshot layout before reading the result.
     static std::unique_ptr<HostBuildResult> prepare(const BuildResult& source, class PinnedPool* pool,
                                                     uint8_t** block);
+    // A snapshot that lies at `base` inside a shared block (SnapshotLayout(sparseNodes, used) from
+    // there), filled by the caller: the results of a batched build share one host block.
+    static std::unique_ptr<HostBuildResult> inBlock(const CloudInfo& cloud, const ProcessingInfo& proc,
+                                                    uint32_t depth, uint32_t denseNodes, uint32_t sparseNodes,
+                                                    uint64_t capacity, uint64_t used,
+                                                    const std::vector<float>& timings,
+                                                    std::shared_ptr<uint8_t> block, uint8_t* base);
 
     const CloudInfo& cloud() const override { return itsCloud; }
     const ProcessingInfo& processing() const override { return itsProc; }
@@ -122,7 +129,8 @@ private:
     uint32_t itsDepth = 0, itsDense = 0, itsM = 0;
     uint64_t itsE = 0, itsUsed = 0;
     std::vector<float> itsTimings;
-    std::shared_ptr<uint8_t> itsBlock;  // the snapshot (SnapshotLayout)
+    std::shared_ptr<uint8_t> itsBlock;  // owner of the memory the snapshot lies in
+    const uint8_t* itsBase = nullptr;   // the snapshot (SnapshotLayout) inside itsBlock
     const uint32_t* itsLut = nullptr;
     const uint8_t* itsOut  = nullptr;
 };

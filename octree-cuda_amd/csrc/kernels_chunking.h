@@ -37,13 +37,13 @@ inline unsigned gridFor(uint64_t n, uint64_t cap = 65536)
     return static_cast<unsigned>(b < cap ? b : cap);
 }
 
-__global__ __launch_bounds__(BLOCK) void kFill32(uint32_t* p, uint64_t n, uint32_t v)
+static __global__ __launch_bounds__(BLOCK) void kFill32(uint32_t* p, uint64_t n, uint32_t v)
 {
     for (uint64_t i = globalThread(); i < n; i += globalStride())
         p[i] = v;
 }
 
-__global__ __launch_bounds__(BLOCK) void kFill64(uint64_t* p, uint64_t n, uint64_t v)
+static __global__ __launch_bounds__(BLOCK) void kFill64(uint64_t* p, uint64_t n, uint64_t v)
 {
     for (uint64_t i = globalThread(); i < n; i += globalStride())
         p[i] = v;
@@ -51,21 +51,24 @@ __global__ __launch_bounds__(BLOCK) void kFill64(uint64_t* p, uint64_t n, uint64
 
 // K2 (counts only): kernelPropagatePointCounts (point_count_propagation.cuh:34-86). Child slot k
 // of parent (x, y, z) is (2x + (k>>2 & 1), 2y + (k>>1 & 1), 2z + (k & 1)).
-__global__ __launch_bounds__(BLOCK) void kPropagate(uint32_t* __restrict__ count, uint32_t offset, uint32_t grid,
+__device__ __forceinline__ void propagateOne(uint32_t idx, uint32_t* __restrict__ count, uint32_t offset,
+                                             uint32_t grid, uint32_t childOffset, uint32_t childGrid)
+{
+    const uint32_t cxy = childGrid * childGrid;
+    uint32_t xy        = grid * grid;
+    uint32_t z = idx / xy, y = (idx - z * xy) / grid, x = (idx - z * xy) % grid;
+    const uint32_t* c = count + childOffset + 2 * z * cxy + 2 * y * childGrid + 2 * x;
+    uint32_t sum      = c[0] + c[1] + c[childGrid] + c[childGrid + 1] + c[cxy] + c[cxy + 1] + c[cxy + childGrid] +
+                   c[cxy + childGrid + 1];
+    count[offset + idx] = sum;
+}
+
+static __global__ __launch_bounds__(BLOCK) void kPropagate(uint32_t* __restrict__ count, uint32_t offset, uint32_t grid,
                                                      uint32_t childOffset, uint32_t childGrid)
 {
-    const uint64_t n  = static_cast<uint64_t>(grid) * grid * grid;
-    const uint32_t cxy = childGrid * childGrid;
+    const uint64_t n = static_cast<uint64_t>(grid) * grid * grid;
     for (uint64_t i = globalThread(); i < n; i += globalStride())
-    {
-        uint32_t idx = static_cast<uint32_t>(i);
-        uint32_t xy  = grid * grid;
-        uint32_t z = idx / xy, y = (idx - z * xy) / grid, x = (idx - z * xy) % grid;
-        const uint32_t* c = count + childOffset + 2 * z * cxy + 2 * y * childGrid + 2 * x;
-        uint32_t sum      = c[0] + c[1] + c[childGrid] + c[childGrid + 1] + c[cxy] + c[cxy + 1] + c[cxy + childGrid] +
-                       c[cxy + childGrid + 1];
-        count[offset + idx] = sum;
-    }
+        propagateOne(static_cast<uint32_t>(i), count, offset, grid, childOffset, childGrid);
 }
 
 struct OccupiedIn
@@ -101,8 +104,8 @@ struct LevelOffsets
 
 // First sparse id of every level: ids ascend with the dense index and a level is one dense range,
 // so it is the number of node keys below the level's dense offset (a lower bound over the m keys).
-__global__ void kLevelStarts(const uint32_t* __restrict__ nodeKey, const uint32_t* __restrict__ m,
-                             LevelOffsets lo, uint32_t* __restrict__ levelStart)
+__device__ __forceinline__ void kLevelStartsBody(const uint32_t* __restrict__ nodeKey, const uint32_t* __restrict__ m,
+                                                 const LevelOffsets& lo, uint32_t* __restrict__ levelStart)
 {
     const uint32_t l = threadIdx.x;
     if (l >= lo.levels)
@@ -117,6 +120,12 @@ __global__ void kLevelStarts(const uint32_t* __restrict__ nodeKey, const uint32_
             b = mid;
     }
     levelStart[l] = a;
+}
+
+static __global__ void kLevelStarts(const uint32_t* __restrict__ nodeKey, const uint32_t* __restrict__ m,
+                             LevelOffsets lo, uint32_t* __restrict__ levelStart)
+{
+    kLevelStartsBody(nodeKey, m, lo, levelStart);
 }
 
 struct LevelParams
@@ -142,18 +151,15 @@ __device__ __forceinline__ uint32_t levelOfKey(uint32_t key, const LevelParams& 
 //   finished  = parent is internal (the root is always finished, A13)
 //   pointCount = internal ? 0 : count
 // and the size of the leaf data block an internal node hands out to its leaf children.
-__global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
-                                                     const uint32_t* __restrict__ nodeCount,
-                                                     const int32_t* __restrict__ denseToSparse,
-                                                     uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
-                                                     int32_t* __restrict__ parent,
-                                                     int32_t* __restrict__ children, uint8_t* __restrict__ flags,
-                                                     uint32_t* __restrict__ blockSize,
-                                                     uint32_t* __restrict__ subtree)
+__device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, const uint32_t* __restrict__ nodeKey,
+                                            const uint32_t* __restrict__ nodeCount,
+                                            const int32_t* __restrict__ denseToSparse,
+                                            uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
+                                            int32_t* __restrict__ parent, int32_t* __restrict__ children,
+                                            uint8_t* __restrict__ flags, uint32_t* __restrict__ blockSize,
+                                            uint32_t* __restrict__ subtree)
 {
-    for (uint64_t i = globalThread(); i < m; i += globalStride())
     {
-        uint32_t id  = static_cast<uint32_t>(i);
         uint32_t key = nodeKey[id];
         uint32_t l   = levelOfKey(key, lp);
         uint32_t g   = lp.grid[l];
@@ -202,6 +208,20 @@ __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, 
     }
 }
 
+static __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
+                                                     const uint32_t* __restrict__ nodeCount,
+                                                     const int32_t* __restrict__ denseToSparse,
+                                                     uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
+                                                     int32_t* __restrict__ parent,
+                                                     int32_t* __restrict__ children, uint8_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ blockSize,
+                                                     uint32_t* __restrict__ subtree)
+{
+    for (uint64_t i = globalThread(); i < m; i += globalStride())
+        nodeLinksOf(static_cast<uint32_t>(i), lp, nodeKey, nodeCount, denseToSparse, pointCount, dataIdx, parent,
+                    children, flags, blockSize, subtree);
+}
+
 struct ArrayIn
 {
     const uint32_t* v;
@@ -216,21 +236,18 @@ struct ExclusiveToArray
 
 // Leaf data offsets (A4): an internal node's block starts at the exclusive scan of block sizes in
 // sparse-id order (= parent level, parent linear index); children follow in slot order.
-__global__ __launch_bounds__(BLOCK) void kLeafDataIdx(uint32_t m, uint32_t threshold,
-                                                       const uint32_t* __restrict__ blockStart,
-                                                       const uint8_t* __restrict__ flags,
-                                                       const int32_t* __restrict__ children,
-                                                       const uint32_t* __restrict__ pointCount,
-                                                       uint64_t* __restrict__ dataIdx)
+__device__ __forceinline__ void leafDataIdxOf(uint32_t id, uint32_t m, const uint32_t* __restrict__ blockStart,
+                                              const uint8_t* __restrict__ flags,
+                                              const int32_t* __restrict__ children,
+                                              const uint32_t* __restrict__ pointCount,
+                                              uint64_t* __restrict__ dataIdx)
 {
-    for (uint64_t i = globalThread(); i < m; i += globalStride())
     {
-        uint32_t id = static_cast<uint32_t>(i);
         if (!(flags[id] & NODE_INTERNAL))
         {
             if (id == m - 1)
                 dataIdx[id] = 0;  // root as the only leaf (A13)
-            continue;
+            return;
         }
         uint64_t r = blockStart[id];
         for (int k = 0; k < 8; ++k)
@@ -245,19 +262,35 @@ __global__ __launch_bounds__(BLOCK) void kLeafDataIdx(uint32_t m, uint32_t thres
     }
 }
 
+static __global__ __launch_bounds__(BLOCK) void kLeafDataIdx(uint32_t m, uint32_t threshold,
+                                                       const uint32_t* __restrict__ blockStart,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const int32_t* __restrict__ children,
+                                                       const uint32_t* __restrict__ pointCount,
+                                                       uint64_t* __restrict__ dataIdx)
+{
+    for (uint64_t i = globalThread(); i < m; i += globalStride())
+        leafDataIdxOf(static_cast<uint32_t>(i), m, blockStart, flags, children, pointCount, dataIdx);
+}
+
 // Level-0 cell -> its leaf (the first finished ancestor-or-self, point_distributing.cuh:56-60),
 // written over the level-0 counts so the distribution needs one lookup per point.
-__global__ __launch_bounds__(BLOCK) void kCellLeaf(uint32_t level0Nodes, const uint32_t* __restrict__ nodeKey,
+__device__ __forceinline__ void cellLeafOf(uint32_t i, const uint32_t* __restrict__ nodeKey,
+                                           const uint8_t* __restrict__ flags, const int32_t* __restrict__ parent,
+                                           uint32_t* __restrict__ count)
+{
+    int32_t s = static_cast<int32_t>(i);
+    while (!(flags[s] & NODE_FINISHED))
+        s = parent[s];
+    count[nodeKey[i]] = static_cast<uint32_t>(s);
+}
+
+static __global__ __launch_bounds__(BLOCK) void kCellLeaf(uint32_t level0Nodes, const uint32_t* __restrict__ nodeKey,
                                                     const uint8_t* __restrict__ flags,
                                                     const int32_t* __restrict__ parent, uint32_t* __restrict__ count)
 {
     for (uint64_t i = globalThread(); i < level0Nodes; i += globalStride())
-    {
-        int32_t s = static_cast<int32_t>(i);
-        while (!(flags[s] & NODE_FINISHED))
-            s = parent[s];
-        count[nodeKey[i]] = static_cast<uint32_t>(s);
-    }
+        cellLeafOf(static_cast<uint32_t>(i), nodeKey, flags, parent, count);
 }
 
 }  // namespace oc

@@ -136,7 +136,7 @@ __device__ __forceinline__ uint32_t pointMorton(double x, double y, double z, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(HIST_THREADS) void kBucketHist(const uint8_t* __restrict__ cloud, uint32_t n,
+static __global__ __launch_bounds__(HIST_THREADS) void kBucketHist(const uint8_t* __restrict__ cloud, uint32_t n,
                                                              uint32_t stride, Gridding g, uint32_t fineBits,
                                                              uint32_t buckets, uint32_t* __restrict__ bucketCount,
                                                              uint32_t* __restrict__ err)
@@ -182,22 +182,32 @@ struct BucketStartOut
 // reserves one range per occupied bucket with a single global atomic, stages the records in LDS
 // in bucket order and writes them out so that consecutive lanes write consecutive records of one
 // bucket.
+// LDS of one bucket-scatter workgroup
 template <typename T>
-__global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
-                                                                uint32_t stride, Gridding g, uint32_t fineBits,
-                                                                uint32_t buckets, uint32_t* __restrict__ cursor,
-                                                                SortedPoint<T>* __restrict__ dst,
-                                                                uint16_t* __restrict__ fineCode)
+struct ScatterLds
 {
-    __shared__ SortedPoint<T> stage[STAGE_CHUNK];
-    __shared__ uint32_t delta[MAX_BUCKETS];           // counts, then global start - local start
-    __shared__ uint16_t localStart[MAX_BUCKETS + 1];  // first staged slot of every bucket
-    __shared__ uint32_t scratch[SORT_WAVES];
+    SortedPoint<T> stage[STAGE_CHUNK];
+    uint32_t delta[MAX_BUCKETS];           // counts, then global start - local start
+    uint16_t localStart[MAX_BUCKETS + 1];  // first staged slot of every bucket
+    uint32_t scratch[SORT_WAVES];
+};
+
+// One chunk of STAGE_CHUNK points (all threads of the workgroup; ends with the LDS free again).
+template <typename T>
+__device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t* __restrict__ cloud, uint32_t n,
+                                                   uint32_t stride, const Gridding& g, uint32_t fineBits,
+                                                   uint32_t* __restrict__ cursor, SortedPoint<T>* __restrict__ dst,
+                                                   uint16_t* __restrict__ fineCode, ScatterLds<T>& lds)
+{
+    SortedPoint<T>* stage = lds.stage;
+    uint32_t* delta       = lds.delta;
+    uint16_t* localStart  = lds.localStart;
+    uint32_t* scratch     = lds.scratch;
     for (uint32_t b = threadIdx.x; b < MAX_BUCKETS; b += SORT_THREADS)
         delta[b] = 0;
     __syncthreads();
 
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * STAGE_CHUNK;
+    const uint64_t base = static_cast<uint64_t>(chunk) * STAGE_CHUNK;
     uint32_t code[STAGE_ITEMS], rank[STAGE_ITEMS];
     SortedPoint<T> p[STAGE_ITEMS];
 #pragma unroll
@@ -278,6 +288,18 @@ __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __
             continue;  // cannot happen: the histogram and this pass compute the same codes
         dst[d] = stage[j];
     }
+    __syncthreads();
+}
+
+template <typename T>
+static __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
+                                                                uint32_t stride, Gridding g, uint32_t fineBits,
+                                                                uint32_t buckets, uint32_t* __restrict__ cursor,
+                                                                SortedPoint<T>* __restrict__ dst,
+                                                                uint16_t* __restrict__ fineCode)
+{
+    __shared__ ScatterLds<T> lds;
+    bucketScatterChunk<T>(blockIdx.x, cloud, n, stride, g, fineBits, cursor, dst, fineCode, lds);
 }
 
 struct FineSortParams
@@ -390,13 +412,12 @@ __device__ __forceinline__ void finePermute(const uint16_t* __restrict__ fineCod
     }
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
-                                                           const uint32_t* __restrict__ bucketStart, FineSortParams fp,
-                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
+// One bucket (all threads of the workgroup; h holds MAX_FINE_CELLS + 1 values). Ends with a barrier.
+__device__ __forceinline__ void fineSortBucket(uint32_t b, const uint16_t* __restrict__ fineCode,
+                                               const uint32_t* __restrict__ bucketStart, const FineSortParams& fp,
+                                               uint32_t* __restrict__ count, uint32_t* __restrict__ perm, uint32_t* h,
+                                               uint32_t* scratch)
 {
-    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
-    __shared__ uint32_t scratch[SORT_WAVES];
-    const uint32_t b     = blockIdx.x;
     const uint32_t cells = 1u << (3 * fp.lc);
     const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
     if (hi - lo > FINE_SPLIT)
@@ -419,6 +440,15 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __rest
 
     // the permutation into Morton order: 4-byte scattered stores instead of 32-byte records
     finePermute(fineCode, lo, hi, h, lo, perm);
+}
+
+static __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
+                                                           const uint32_t* __restrict__ bucketStart, FineSortParams fp,
+                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
+    __shared__ uint32_t scratch[SORT_WAVES];
+    fineSortBucket(blockIdx.x, fineCode, bucketStart, fp, count, perm, h, scratch);
 }
 
 // chunks of the buckets above FINE_SPLIT points: entry = first chunk of the bucket's points,
@@ -444,17 +474,13 @@ struct SplitChunksOut
 };
 
 // Split path, 1: the histogram of one chunk over the bucket's fine cells, to chunkHist[chunk].
-__global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint16_t* __restrict__ fineCode,
-                                                                const uint32_t* __restrict__ bucketStart,
-                                                                const uint64_t* __restrict__ chunks,
-                                                                const uint32_t* __restrict__ chunkCount, uint32_t lc,
-                                                                uint32_t* __restrict__ chunkHist)
+__device__ __forceinline__ void fineChunkHist(uint32_t chunk, const uint16_t* __restrict__ fineCode,
+                                              const uint32_t* __restrict__ bucketStart,
+                                              const uint64_t* __restrict__ chunks, uint32_t lc,
+                                              uint32_t* __restrict__ chunkHist, uint32_t* h)
 {
-    __shared__ uint32_t h[MAX_FINE_CELLS];
-    if (blockIdx.x >= *chunkCount)
-        return;  // the whole workgroup
     const uint32_t cells = 1u << (3 * lc);
-    const uint64_t e     = chunks[blockIdx.x];
+    const uint64_t e     = chunks[chunk];
     const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
     const uint32_t lo = bucketStart[b] + part * FINE_SPLIT;
     const uint32_t hi = min(bucketStart[b + 1], lo + FINE_SPLIT);
@@ -463,29 +489,37 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint16_t* _
     __syncthreads();
     fineHistogram(fineCode, lo, hi, h);
     __syncthreads();
-    uint32_t* dst = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
+    uint32_t* dst = chunkHist + static_cast<uint64_t>(chunk) * MAX_FINE_CELLS;
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
         dst[c] = h[c];
+}
+
+static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint16_t* __restrict__ fineCode,
+                                                                const uint32_t* __restrict__ bucketStart,
+                                                                const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ chunkCount, uint32_t lc,
+                                                                uint32_t* __restrict__ chunkHist)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS];
+    if (blockIdx.x >= *chunkCount)
+        return;  // the whole workgroup
+    fineChunkHist(blockIdx.x, fineCode, bucketStart, chunks, lc, chunkHist, h);
 }
 
 // Split path, 2: per split bucket (the workgroup of its part-0 chunk), the bucket's histogram as
 // the sum of its chunks', its scan and dense counts, and each chunk's starting slot per fine cell
 // (the cell's prefix plus the cell's points in the bucket's earlier chunks), in place of the
 // chunk histograms.
-__global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint32_t* __restrict__ bucketStart,
-                                                                const uint64_t* __restrict__ chunks,
-                                                                const uint32_t* __restrict__ chunkCount,
-                                                                FineSortParams fp, uint32_t* __restrict__ count,
-                                                                uint32_t* __restrict__ chunkHist)
+// (all threads; chunk is the first chunk of its bucket)
+__device__ __forceinline__ void fineSplitScan(uint32_t chunk, const uint32_t* __restrict__ bucketStart,
+                                              const uint64_t* __restrict__ chunks, const FineSortParams& fp,
+                                              uint32_t* __restrict__ count, uint32_t* __restrict__ chunkHist,
+                                              uint32_t* h, uint32_t* scratch)
 {
-    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
-    __shared__ uint32_t scratch[SORT_WAVES];
-    if (blockIdx.x >= *chunkCount || static_cast<uint32_t>(chunks[blockIdx.x]) != 0)
-        return;  // the whole workgroup: not the first chunk of a bucket
-    const uint32_t b     = static_cast<uint32_t>(chunks[blockIdx.x] >> 32);
+    const uint32_t b     = static_cast<uint32_t>(chunks[chunk] >> 32);
     const uint32_t parts = (bucketStart[b + 1] - bucketStart[b] + FINE_SPLIT - 1) / FINE_SPLIT;
     const uint32_t cells = 1u << (3 * fp.lc);
-    uint32_t* first      = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
+    uint32_t* first      = chunkHist + static_cast<uint64_t>(chunk) * MAX_FINE_CELLS;
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
     {
         uint32_t v = 0;
@@ -509,8 +543,40 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint32_t* _
     }
 }
 
+static __global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint32_t* __restrict__ bucketStart,
+                                                                const uint64_t* __restrict__ chunks,
+                                                                const uint32_t* __restrict__ chunkCount,
+                                                                FineSortParams fp, uint32_t* __restrict__ count,
+                                                                uint32_t* __restrict__ chunkHist)
+{
+    __shared__ uint32_t h[MAX_FINE_CELLS + 1];
+    __shared__ uint32_t scratch[SORT_WAVES];
+    if (blockIdx.x >= *chunkCount || static_cast<uint32_t>(chunks[blockIdx.x]) != 0)
+        return;  // the whole workgroup: not the first chunk of a bucket
+    fineSplitScan(blockIdx.x, bucketStart, chunks, fp, count, chunkHist, h, scratch);
+}
+
 // Split path, 3: one chunk's part of the permutation from its starting slots.
-__global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* __restrict__ fineCode,
+__device__ __forceinline__ void fineChunkPerm(uint32_t chunk, const uint16_t* __restrict__ fineCode,
+                                              const uint32_t* __restrict__ bucketStart,
+                                              const uint64_t* __restrict__ chunks, uint32_t lc,
+                                              const uint32_t* __restrict__ chunkHist, uint32_t* __restrict__ perm,
+                                              uint32_t* h)
+{
+    const uint32_t cells = 1u << (3 * lc);
+    const uint64_t e     = chunks[chunk];
+    const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
+    const uint32_t base = bucketStart[b];
+    const uint32_t lo   = base + part * FINE_SPLIT;
+    const uint32_t hi   = min(bucketStart[b + 1], lo + FINE_SPLIT);
+    const uint32_t* src = chunkHist + static_cast<uint64_t>(chunk) * MAX_FINE_CELLS;
+    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+        h[c] = src[c];
+    __syncthreads();
+    finePermute(fineCode, lo, hi, h, base, perm);
+}
+
+static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* __restrict__ fineCode,
                                                                 const uint32_t* __restrict__ bucketStart,
                                                                 const uint64_t* __restrict__ chunks,
                                                                 const uint32_t* __restrict__ chunkCount, uint32_t lc,
@@ -520,17 +586,7 @@ __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* _
     __shared__ uint32_t h[MAX_FINE_CELLS];
     if (blockIdx.x >= *chunkCount)
         return;  // the whole workgroup
-    const uint32_t cells = 1u << (3 * lc);
-    const uint64_t e     = chunks[blockIdx.x];
-    const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
-    const uint32_t base = bucketStart[b];
-    const uint32_t lo   = base + part * FINE_SPLIT;
-    const uint32_t hi   = min(bucketStart[b + 1], lo + FINE_SPLIT);
-    const uint32_t* src = chunkHist + static_cast<uint64_t>(blockIdx.x) * MAX_FINE_CELLS;
-    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
-        h[c] = src[c];
-    __syncthreads();
-    finePermute(fineCode, lo, hi, h, base, perm);
+    fineChunkPerm(blockIdx.x, fineCode, bucketStart, chunks, lc, chunkHist, perm, h);
 }
 
 // The sorted cloud: gathered records, written in order. The permutation only moves a record
@@ -550,7 +606,7 @@ inline unsigned gatherGrid(uint32_t n)
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void kGatherSorted(const SortedPoint<T>* __restrict__ bucketed,
+static __global__ __launch_bounds__(BLOCK) void kGatherSorted(const SortedPoint<T>* __restrict__ bucketed,
                                                         const uint32_t* __restrict__ perm, uint32_t n,
                                                         SortedPoint<T>* __restrict__ sorted)
 {
@@ -568,7 +624,38 @@ __global__ __launch_bounds__(BLOCK) void kGatherSorted(const SortedPoint<T>* __r
 
 // Start of every leaf's range in the sorted cloud: the points of all nodes that precede it in
 // Morton order, i.e. of the lower-slot siblings of the leaf and of each of its ancestors.
-__global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
+__device__ __forceinline__ void sortedStartOf(uint32_t id, const LevelParams& lp, const uint32_t* __restrict__ nodeKey,
+                                              const uint32_t* __restrict__ nodeCount,
+                                              const uint8_t* __restrict__ flags, const int32_t* __restrict__ parent,
+                                              const int32_t* __restrict__ children,
+                                              uint32_t* __restrict__ sortedStart)
+{
+    if ((flags[id] & (NODE_FINISHED | NODE_INTERNAL)) != NODE_FINISHED)
+        return;
+    uint32_t start = 0;
+    int32_t cur    = static_cast<int32_t>(id);
+    int32_t p      = parent[cur];
+    while (p >= 0)
+    {
+        const uint32_t key = nodeKey[cur];
+        const uint32_t l   = levelOfKey(key, lp);
+        const uint32_t g   = lp.grid[l];
+        const uint32_t lin = key - lp.offset[l];
+        const uint32_t x = lin % g, y = (lin / g) % g, z = lin / (g * g);
+        const int slot     = static_cast<int>(((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
+        for (int k = 0; k < slot; ++k)
+        {
+            const int32_t c = children[8 * static_cast<uint64_t>(p) + k];
+            if (c >= 0)
+                start += nodeCount[c];
+        }
+        cur = p;
+        p   = parent[cur];
+    }
+    sortedStart[id] = start;
+}
+
+static __global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
                                                        const uint32_t* __restrict__ nodeCount,
                                                        const uint8_t* __restrict__ flags,
                                                        const int32_t* __restrict__ parent,
@@ -576,32 +663,7 @@ __global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp
                                                        uint32_t* __restrict__ sortedStart)
 {
     for (uint64_t i = globalThread(); i < m; i += globalStride())
-    {
-        const uint32_t id = static_cast<uint32_t>(i);
-        if ((flags[id] & (NODE_FINISHED | NODE_INTERNAL)) != NODE_FINISHED)
-            continue;
-        uint32_t start = 0;
-        int32_t cur    = static_cast<int32_t>(id);
-        int32_t p      = parent[cur];
-        while (p >= 0)
-        {
-            const uint32_t key = nodeKey[cur];
-            const uint32_t l   = levelOfKey(key, lp);
-            const uint32_t g   = lp.grid[l];
-            const uint32_t lin = key - lp.offset[l];
-            const uint32_t x = lin % g, y = (lin / g) % g, z = lin / (g * g);
-            const int slot     = static_cast<int>(((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
-            for (int k = 0; k < slot; ++k)
-            {
-                const int32_t c = children[8 * static_cast<uint64_t>(p) + k];
-                if (c >= 0)
-                    start += nodeCount[c];
-            }
-            cur = p;
-            p   = parent[cur];
-        }
-        sortedStart[id] = start;
-    }
+        sortedStartOf(static_cast<uint32_t>(i), lp, nodeKey, nodeCount, flags, parent, children, sortedStart);
 }
 
 // K5: kernelDistributePoints (point_distributing.cuh:31-76) over the sorted cloud. The leaf of a
@@ -610,7 +672,7 @@ __global__ __launch_bounds__(BLOCK) void kSortedStart(uint32_t m, LevelParams lp
 constexpr int DIST_ITEMS = 4;
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void kDistributeSorted(const SortedPoint<T>* __restrict__ sorted, uint32_t n,
+static __global__ __launch_bounds__(BLOCK) void kDistributeSorted(const SortedPoint<T>* __restrict__ sorted, uint32_t n,
                                                             Gridding g, const uint32_t* __restrict__ cellLeaf,
                                                             const uint32_t* __restrict__ sortedStart,
                                                             const uint64_t* __restrict__ dataIdx,

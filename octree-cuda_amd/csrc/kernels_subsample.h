@@ -214,7 +214,7 @@ __device__ __forceinline__ void segmentsOf(uint64_t k, const SubsampleParams& sp
     lt.nodeGrid[k] = ng;
 }
 
-__global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp, LevelTables lt)
+static __global__ __launch_bounds__(BLOCK) void kSegments(uint32_t nodes, SubsampleParams sp, LevelTables lt)
 {
     for (uint64_t k = globalThread(); k < nodes; k += globalStride())
         segmentsOf(k, sp, lt);
@@ -303,19 +303,25 @@ struct LeafKeyer
     }
 };
 
-template <typename T>
-__global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* __restrict__ chunkList, LeafOrderParams p,
-                                                                const uint32_t* __restrict__ nodeKey,
-                                                                const int32_t* __restrict__ parent,
-                                                                const uint32_t* __restrict__ count,
-                                                                const uint32_t* __restrict__ sortedStart,
-                                                                const SortedPoint<T>* __restrict__ bucketed,
-                                                                const uint32_t* __restrict__ perm,
-                                                                SortedPoint<T>* __restrict__ out)
+struct LeafOrderLds
 {
-    __shared__ uint32_t hist[LEAF_KEYS + LEAF_KEYS / 32];
-    __shared__ uint32_t scanLds[LEAF_THREADS / 64];
-    const uint64_t entry = chunkList[blockIdx.x];
+    uint32_t hist[LEAF_KEYS + LEAF_KEYS / 32];
+    uint32_t scanLds[LEAF_THREADS / 64];
+};
+
+// One chunk of one leaf (all threads; the LDS is free again on return).
+template <typename T>
+__device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrderParams& p,
+                                                   const uint32_t* __restrict__ nodeKey,
+                                                   const int32_t* __restrict__ parent,
+                                                   const uint32_t* __restrict__ count,
+                                                   const uint32_t* __restrict__ sortedStart,
+                                                   const SortedPoint<T>* __restrict__ bucketed,
+                                                   const uint32_t* __restrict__ perm, SortedPoint<T>* __restrict__ out,
+                                                   LeafOrderLds& lds)
+{
+    uint32_t* hist    = lds.hist;
+    uint32_t* scanLds = lds.scanLds;
     const uint32_t leaf  = static_cast<uint32_t>(entry);
     const uint32_t first = static_cast<uint32_t>(entry >> 32) * LEAF_CHUNK;
     const uint32_t total = count[leaf];
@@ -390,6 +396,21 @@ __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* _
     for (int k = 0; k < LEAF_ITEMS; ++k)
         if (threadIdx.x + k * LEAF_THREADS < cnt)
             out[src + hist[leafSlot(key[k])] + rank[k]] = rec[k];
+    __syncthreads();
+}
+
+template <typename T>
+static __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint64_t* __restrict__ chunkList, LeafOrderParams p,
+                                                                const uint32_t* __restrict__ nodeKey,
+                                                                const int32_t* __restrict__ parent,
+                                                                const uint32_t* __restrict__ count,
+                                                                const uint32_t* __restrict__ sortedStart,
+                                                                const SortedPoint<T>* __restrict__ bucketed,
+                                                                const uint32_t* __restrict__ perm,
+                                                                SortedPoint<T>* __restrict__ out)
+{
+    __shared__ LeafOrderLds lds;
+    leafCellOrderChunk<T>(chunkList[blockIdx.x], p, nodeKey, parent, count, sortedStart, bucketed, perm, out, lds);
 }
 
 // packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums
@@ -414,16 +435,20 @@ struct SegmentOut
     }
 };
 
-__global__ __launch_bounds__(BLOCK) void kTileList(uint64_t segments, const uint32_t* __restrict__ segCount,
+__device__ __forceinline__ void tileListOf(uint64_t s, const uint32_t* __restrict__ segCount,
+                                           const uint32_t* __restrict__ segTile, uint32_t* __restrict__ tileList)
+{
+    uint32_t tiles = (segCount[s] + TILE - 1) / TILE;
+    uint32_t first = segTile[s];
+    for (uint32_t t = 0; t < tiles; ++t)
+        tileList[first + t] = static_cast<uint32_t>(s);
+}
+
+static __global__ __launch_bounds__(BLOCK) void kTileList(uint64_t segments, const uint32_t* __restrict__ segCount,
                                                     const uint32_t* __restrict__ segTile, uint32_t* __restrict__ tileList)
 {
     for (uint64_t s = globalThread(); s < segments; s += globalStride())
-    {
-        uint32_t tiles = (segCount[s] + TILE - 1) / TILE;
-        uint32_t first = segTile[s];
-        for (uint32_t t = 0; t < tiles; ++t)
-            tileList[first + t] = static_cast<uint32_t>(s);
-    }
+        tileListOf(s, segCount, segTile, tileList);
 }
 
 // A whole level's setup in one workgroup and one launch, for small levels (up to
@@ -433,7 +458,7 @@ __global__ __launch_bounds__(BLOCK) void kTileList(uint64_t segments, const uint
 constexpr uint32_t SMALL_LEVEL_NODES  = 1u << 16;
 constexpr uint64_t SMALL_LEVEL_POINTS = 1u << 22;
 
-__global__ __launch_bounds__(SCAN_THREADS) void kLevelSetup(uint32_t levelStart, uint32_t levelNodes,
+static __global__ __launch_bounds__(SCAN_THREADS) void kLevelSetup(uint32_t levelStart, uint32_t levelNodes,
                                                             SubsampleParams sp, LevelTables lt,
                                                             uint32_t* __restrict__ internalTotal,
                                                             uint64_t* __restrict__ segTotal)
@@ -483,18 +508,15 @@ __device__ __forceinline__ InputRef inputOf(uint32_t tile, const uint32_t* tileL
 // Evaluate, part 1 (rp_subsample_evaluation.cuh:37-203, fp_subsample_evaluation.cuh:22-110):
 // the cell of every input point in its node's grid; marks the cell occupied.
 template <typename T>
-__global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
-                                               const uint32_t* __restrict__ tileList,
-                                               const uint32_t* __restrict__ segTile,
-                                               const uint32_t* __restrict__ segCount,
-                                               const uint32_t* __restrict__ segSrc,
-                                               const uint32_t* __restrict__ segInput,
-                                               const NodeGrid* __restrict__ nodeGrid, PointStore<T> ps,
-                                               uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
-                                               uint32_t* __restrict__ err)
+__device__ __forceinline__ void markTile(uint32_t tile, uint32_t nodeBase, uint64_t wordsPerNode,
+                                         const uint32_t* __restrict__ tileList, const uint32_t* __restrict__ segTile,
+                                         const uint32_t* __restrict__ segCount, const uint32_t* __restrict__ segSrc,
+                                         const uint32_t* __restrict__ segInput, const NodeGrid* __restrict__ nodeGrid,
+                                         const PointStore<T>& ps, uint32_t* __restrict__ bitmap,
+                                         uint32_t* __restrict__ cellOf, uint32_t* __restrict__ err)
 {
     // All lanes take part in the wave-level merge of bit sets bound for the same bitmap word.
-    InputRef in   = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    InputRef in   = inputOf(tile, tileList, segTile, segCount, segSrc, segInput);
     bool valid    = in.valid;
     uint32_t cell = ~0u;
     if (valid)
@@ -525,6 +547,21 @@ __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBa
         atomicOr(bitmap + word, bits);
 }
 
+template <typename T>
+static __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
+                                               const uint32_t* __restrict__ tileList,
+                                               const uint32_t* __restrict__ segTile,
+                                               const uint32_t* __restrict__ segCount,
+                                               const uint32_t* __restrict__ segSrc,
+                                               const uint32_t* __restrict__ segInput,
+                                               const NodeGrid* __restrict__ nodeGrid, PointStore<T> ps,
+                                               uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
+                                               uint32_t* __restrict__ err)
+{
+    markTile<T>(tileBase + blockIdx.x, nodeBase, wordsPerNode, tileList, segTile, segCount, segSrc, segInput,
+                nodeGrid, ps, bitmap, cellOf, err);
+}
+
 struct PopcountIn
 {
     const uint32_t* bitmap;
@@ -539,26 +576,33 @@ __device__ __forceinline__ uint32_t cellRank(const uint32_t* bitmap, const uint3
 }
 
 // Per node of the batch: subsample size (occupied cells), staging position, first dense index.
-__global__ __launch_bounds__(BLOCK) void kBatchNodes(uint32_t nodeBase, uint32_t nodes, uint64_t wordsPerNode,
+__device__ __forceinline__ void batchNodeOf(uint64_t kl, uint32_t nodeBase, uint32_t nodes, uint64_t wordsPerNode,
+                                            const NodeGrid* __restrict__ nodeGrid,
+                                            const uint32_t* __restrict__ prefix, uint32_t cursor,
+                                            uint32_t* __restrict__ pointCount, uint32_t* __restrict__ stage,
+                                            uint32_t* __restrict__ nodeFirst)
+{
+    uint32_t b    = prefix[kl * wordsPerNode];
+    nodeFirst[kl] = b;
+    if (kl == nodes)
+        return;
+    uint32_t e     = prefix[(kl + 1) * wordsPerNode];
+    uint32_t id    = nodeGrid[nodeBase + kl].id;
+    pointCount[id] = e - b;
+    stage[id]      = cursor + b;
+}
+
+static __global__ __launch_bounds__(BLOCK) void kBatchNodes(uint32_t nodeBase, uint32_t nodes, uint64_t wordsPerNode,
                                                       const NodeGrid* __restrict__ nodeGrid,
                                                       const uint32_t* __restrict__ prefix, uint32_t cursor,
                                                       uint32_t* __restrict__ pointCount, uint32_t* __restrict__ stage,
                                                       uint32_t* __restrict__ nodeFirst)
 {
     for (uint64_t kl = globalThread(); kl <= nodes; kl += globalStride())
-    {
-        uint32_t b    = prefix[kl * wordsPerNode];
-        nodeFirst[kl] = b;
-        if (kl == nodes)
-            continue;
-        uint32_t e     = prefix[(kl + 1) * wordsPerNode];
-        uint32_t id    = nodeGrid[nodeBase + kl].id;
-        pointCount[id] = e - b;
-        stage[id]      = cursor + b;
-    }
+        batchNodeOf(kl, nodeBase, nodes, wordsPerNode, nodeGrid, prefix, cursor, pointCount, stage, nodeFirst);
 }
 
-__global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __restrict__ win,
+static __global__ __launch_bounds__(BLOCK) void kClearCompact(uint64_t n, uint64_t* __restrict__ win,
                                                         uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt)
 {
     for (uint64_t i = globalThread(); i < n; i += globalStride())
@@ -603,23 +647,22 @@ struct PrefixClearOut
 // colour sum of the input records (intra-cell, and the per-cell sums the inter-cell filter is
 // built from), and per-cell point counts + each point's rank in its cell (weighted filter).
 template <typename T, uint32_t MODE, bool RANDOM>
-__global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
-                                                     const uint32_t* __restrict__ tileList,
-                                                     const uint32_t* __restrict__ segTile,
-                                                     const uint32_t* __restrict__ segCount,
-                                                     const uint32_t* __restrict__ segSrc,
-                                                     const uint32_t* __restrict__ segInput,
-                                                     const NodeGrid* __restrict__ nodeGrid,
-                                                     const uint32_t* __restrict__ bitmap,
-                                                     const uint32_t* __restrict__ prefix,
-                                                     uint32_t* __restrict__ cellOf, PointStore<T> ps,
-                                                     uint64_t* __restrict__ win,
-                                                     uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
-                                                     uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
-                                                     uint32_t* __restrict__ rankOf)
+__device__ __forceinline__ void accumulateTile(uint32_t tile, uint32_t nodeBase, uint64_t wordsPerNode,
+                                               const uint32_t* __restrict__ tileList,
+                                               const uint32_t* __restrict__ segTile,
+                                               const uint32_t* __restrict__ segCount,
+                                               const uint32_t* __restrict__ segSrc,
+                                               const uint32_t* __restrict__ segInput,
+                                               const NodeGrid* __restrict__ nodeGrid,
+                                               const uint32_t* __restrict__ bitmap,
+                                               const uint32_t* __restrict__ prefix, uint32_t* __restrict__ cellOf,
+                                               const PointStore<T>& ps, uint64_t* __restrict__ win,
+                                               uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
+                                               uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
+                                               uint32_t* __restrict__ rankOf)
 {
     // All lanes take part: runs of lanes in the same cell make one atomic per run.
-    InputRef in         = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    InputRef in         = inputOf(tile, tileList, segTile, segCount, segSrc, segInput);
     const uint32_t cell = in.valid ? cellOf[in.q] : ~0u;
     const bool valid    = cell != ~0u;  // an invalid cell: the point lay outside its node's grid
     uint32_t c          = 0;
@@ -672,6 +715,27 @@ __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t 
     }
 }
 
+template <typename T, uint32_t MODE, bool RANDOM>
+static __global__ __launch_bounds__(TILE) void kAccumulate(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                     const uint32_t* __restrict__ tileList,
+                                                     const uint32_t* __restrict__ segTile,
+                                                     const uint32_t* __restrict__ segCount,
+                                                     const uint32_t* __restrict__ segSrc,
+                                                     const uint32_t* __restrict__ segInput,
+                                                     const NodeGrid* __restrict__ nodeGrid,
+                                                     const uint32_t* __restrict__ bitmap,
+                                                     const uint32_t* __restrict__ prefix,
+                                                     uint32_t* __restrict__ cellOf, PointStore<T> ps,
+                                                     uint64_t* __restrict__ win,
+                                                     uint64_t* __restrict__ color, uint32_t* __restrict__ cellCnt,
+                                                     uint32_t* __restrict__ cellKey, uint32_t* __restrict__ cellNode,
+                                                     uint32_t* __restrict__ rankOf)
+{
+    accumulateTile<T, MODE, RANDOM>(tileBase + blockIdx.x, nodeBase, wordsPerNode, tileList, segTile, segCount,
+                                    segSrc, segInput, nodeGrid, bitmap, prefix, cellOf, ps, win, color, cellCnt,
+                                    cellKey, cellNode, rankOf);
+}
+
 // Per-point data of the weighted filter, in cell order so that a cell's points are one contiguous
 // run (cellStart[cell] + arrival rank), in two arrays:
 //   WeightPayload: the point's offset from its own cell's centre, (p - centre) rounded to float,
@@ -699,20 +763,19 @@ __device__ __forceinline__ void cellCentre(const Gridding& g, uint32_t ix, uint3
 }
 
 template <typename T, bool RANDOM>
-__global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
-                                                         const uint32_t* __restrict__ segTile,
-                                                         const uint32_t* __restrict__ segCount,
-                                                         const uint32_t* __restrict__ segSrc,
-                                                         const uint32_t* __restrict__ segInput,
-                                                         const NodeGrid* __restrict__ nodeGrid,
-                                                         const uint32_t* __restrict__ cellKey,
-                                                         const uint32_t* __restrict__ rankOf,
-                                                         const uint32_t* __restrict__ cellStart,
-                                                         const uint32_t* __restrict__ cellOf, PointStore<T> ps,
-                                                         WeightPayload* __restrict__ payload,
-                                                         uint64_t* __restrict__ winKey)
+__device__ __forceinline__ void scatterPayloadTile(uint32_t tile, const uint32_t* __restrict__ tileList,
+                                                   const uint32_t* __restrict__ segTile,
+                                                   const uint32_t* __restrict__ segCount,
+                                                   const uint32_t* __restrict__ segSrc,
+                                                   const uint32_t* __restrict__ segInput,
+                                                   const NodeGrid* __restrict__ nodeGrid,
+                                                   const uint32_t* __restrict__ cellKey,
+                                                   const uint32_t* __restrict__ rankOf,
+                                                   const uint32_t* __restrict__ cellStart,
+                                                   const uint32_t* __restrict__ cellOf, const PointStore<T>& ps,
+                                                   WeightPayload* __restrict__ payload, uint64_t* __restrict__ winKey)
 {
-    InputRef in = inputOf(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput);
+    InputRef in = inputOf(tile, tileList, segTile, segCount, segSrc, segInput);
     if (!in.valid)
         return;
     const uint32_t c = cellOf[in.q];  // the dense index kAccumulate left there
@@ -740,6 +803,24 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
     winKey[slot]        = (static_cast<uint64_t>(rank) << 32) | in.pos;
 }
 
+template <typename T, bool RANDOM>
+static __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
+                                                         const uint32_t* __restrict__ segTile,
+                                                         const uint32_t* __restrict__ segCount,
+                                                         const uint32_t* __restrict__ segSrc,
+                                                         const uint32_t* __restrict__ segInput,
+                                                         const NodeGrid* __restrict__ nodeGrid,
+                                                         const uint32_t* __restrict__ cellKey,
+                                                         const uint32_t* __restrict__ rankOf,
+                                                         const uint32_t* __restrict__ cellStart,
+                                                         const uint32_t* __restrict__ cellOf, PointStore<T> ps,
+                                                         WeightPayload* __restrict__ payload,
+                                                         uint64_t* __restrict__ winKey)
+{
+    scatterPayloadTile<T, RANDOM>(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput, nodeGrid,
+                                  cellKey, rankOf, cellStart, cellOf, ps, payload, winKey);
+}
+
 // Select (rp_subsampling.cuh:70-321, fp_subsampling.cuh:30-306): one output record per occupied
 // cell, written to staging at its dense index (node by node, cells ascending). The record keeps the
 // winner's global quantised position; the colour is the winner's (no filter), the decoded intra
@@ -748,18 +829,13 @@ __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const
 // neighbourhood is walked as 9 x-rows: the occupied cells among x-1..x+1 of one row have
 // consecutive dense indices, so each row is one contiguous run of per-cell sums.
 template <typename T, uint32_t MODE>
-__global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
-                                                uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
-                                                const uint32_t* __restrict__ bitmap,
-                                                const uint32_t* __restrict__ prefix,
-                                                const uint32_t* __restrict__ cellKey,
-                                                const uint32_t* __restrict__ cellNode,
-                                                const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
-                                                PointStore<T> ps)
+__device__ __forceinline__ void emitCell(uint32_t c, uint32_t nodeBase, uint64_t wordsPerNode, uint32_t cursor,
+                                         const NodeGrid* __restrict__ nodeGrid, const uint32_t* __restrict__ bitmap,
+                                         const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ cellKey,
+                                         const uint32_t* __restrict__ cellNode, const uint64_t* __restrict__ win,
+                                         const uint64_t* __restrict__ color, const PointStore<T>& ps)
 {
-    for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
     {
-        const uint32_t c  = static_cast<uint32_t>(ci);
         const uint32_t kl = cellNode[c];
         const NodeGrid ng = nodeGrid[nodeBase + kl];
         OutRecord rec{};
@@ -821,6 +897,21 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
     }
 }
 
+template <typename T, uint32_t MODE>
+static __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
+                                                const uint32_t* __restrict__ bitmap,
+                                                const uint32_t* __restrict__ prefix,
+                                                const uint32_t* __restrict__ cellKey,
+                                                const uint32_t* __restrict__ cellNode,
+                                                const uint64_t* __restrict__ win, const uint64_t* __restrict__ color,
+                                                PointStore<T> ps)
+{
+    for (uint64_t ci = globalThread(); ci < cells; ci += globalStride())
+        emitCell<T, MODE>(static_cast<uint32_t>(ci), nodeBase, wordsPerNode, cursor, nodeGrid, bitmap, prefix, cellKey,
+                          cellNode, win, color, ps);
+}
+
 // Select with the distance-weighted inter-cell filter (kernel_inter_cell_filtering.cuh:109-194,
 // weighted decode rp_subsampling.cuh:222-230): the gathered average over the points of the 3x3x3
 // neighbourhood, with the weight of a point p for this cell c
@@ -836,23 +927,16 @@ __global__ __launch_bounds__(BLOCK) void kEmit(uint32_t cells, uint32_t nodeBase
 // same tolerance the float sums' order dependence already needs (the reference adds with
 // atomics). Two records per step, both loads in flight before either is weighed.
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
-                                                        uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
-                                                        const uint32_t* __restrict__ bitmap,
-                                                        const uint32_t* __restrict__ prefix,
-                                                        const uint32_t* __restrict__ cellKey,
-                                                        const uint32_t* __restrict__ cellNode,
-                                                        const uint32_t* __restrict__ cellStart,
-                                                        const WeightPayload* __restrict__ payload,
-                                                        const uint64_t* __restrict__ winKey, PointStore<T> ps)
+__device__ __forceinline__ void emitWeightedCell(uint32_t c, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                 uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
+                                                 const uint32_t* __restrict__ bitmap,
+                                                 const uint32_t* __restrict__ prefix,
+                                                 const uint32_t* __restrict__ cellKey,
+                                                 const uint32_t* __restrict__ cellNode,
+                                                 const uint32_t* __restrict__ cellStart,
+                                                 const WeightPayload* __restrict__ payload,
+                                                 const uint64_t* __restrict__ winKey, const PointStore<T>& ps)
 {
-    // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
-    const uint32_t b     = blockIdx.x;
-    const uint32_t round = (b / 8) / EMIT_CHUNK;
-    const uint32_t block = round * 8 * EMIT_CHUNK + (b % 8) * EMIT_CHUNK + (b / 8) % EMIT_CHUNK;
-    const uint32_t c     = block * BLOCK + threadIdx.x;
-    if (c >= cells)
-        return;
     const uint32_t kl    = cellNode[c];
     const NodeGrid ng    = nodeGrid[nodeBase + kl];
     const uint32_t key   = cellKey[c];
@@ -954,6 +1038,28 @@ __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t 
     d[2]         = bz;
 }
 
+template <typename T>
+static __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, uint32_t nodeBase, uint64_t wordsPerNode,
+                                                        uint32_t cursor, const NodeGrid* __restrict__ nodeGrid,
+                                                        const uint32_t* __restrict__ bitmap,
+                                                        const uint32_t* __restrict__ prefix,
+                                                        const uint32_t* __restrict__ cellKey,
+                                                        const uint32_t* __restrict__ cellNode,
+                                                        const uint32_t* __restrict__ cellStart,
+                                                        const WeightPayload* __restrict__ payload,
+                                                        const uint64_t* __restrict__ winKey, PointStore<T> ps)
+{
+    // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
+    const uint32_t b     = blockIdx.x;
+    const uint32_t round = (b / 8) / EMIT_CHUNK;
+    const uint32_t block = round * 8 * EMIT_CHUNK + (b % 8) * EMIT_CHUNK + (b / 8) % EMIT_CHUNK;
+    const uint32_t c     = block * BLOCK + threadIdx.x;
+    if (c >= cells)
+        return;
+    emitWeightedCell<T>(c, nodeBase, wordsPerNode, cursor, nodeGrid, bitmap, prefix, cellKey, cellNode, cellStart,
+                        payload, winKey, ps);
+}
+
 // DFS post-order placement (A6, kernel_helpers.cuh:55-59): subtree sums of subsample sizes
 // bottom-up, then region starts top-down; an internal node's data follows its children's
 // subtrees: dataIdx = N + regionStart + sum of its children's subtree sizes. kSubtree / kRegion
@@ -994,7 +1100,7 @@ __device__ __forceinline__ void regionOf(uint32_t id, uint32_t root, uint32_t N,
     dataIdx[id] = static_cast<uint64_t>(N) + r;
 }
 
-__global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t nodes,
+static __global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t nodes,
                                                    const uint8_t* __restrict__ flags,
                                                    const int32_t* __restrict__ children,
                                                    const uint32_t* __restrict__ pointCount,
@@ -1004,7 +1110,7 @@ __global__ __launch_bounds__(BLOCK) void kSubtree(uint32_t levelStart, uint32_t 
         subtreeOf(levelStart + static_cast<uint32_t>(j), flags, children, pointCount, subtree);
 }
 
-__global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t nodes, uint32_t root, uint32_t N,
+static __global__ __launch_bounds__(BLOCK) void kRegion(uint32_t levelStart, uint32_t nodes, uint32_t root, uint32_t N,
                                                   const uint8_t* __restrict__ flags,
                                                   const int32_t* __restrict__ children,
                                                   const uint32_t* __restrict__ subtree,
@@ -1024,7 +1130,7 @@ struct PlaceParams
 
 // One workgroup: subtree sizes zeroed, then summed level by level bottom-up, then regions and data
 // offsets top-down, a barrier between levels (the workgroup's writes are visible to it after one).
-__global__ __launch_bounds__(PLACE_THREADS) void kPlaceLevels(PlaceParams pp, const uint8_t* __restrict__ flags,
+static __global__ __launch_bounds__(PLACE_THREADS) void kPlaceLevels(PlaceParams pp, const uint8_t* __restrict__ flags,
                                                               const int32_t* __restrict__ children,
                                                               const uint32_t* __restrict__ pointCount,
                                                               uint32_t* __restrict__ subtree,
@@ -1065,17 +1171,14 @@ struct PlacedChunksIn
 };
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void kPermute(const uint64_t* __restrict__ chunkList, uint32_t N,
-                                                   const uint32_t* __restrict__ stage,
-                                                   const uint64_t* __restrict__ dataIdx,
-                                                   const uint32_t* __restrict__ pointCount,
-                                                   const uint32_t* __restrict__ stLut,
-                                                   const OutRecord* __restrict__ stOut, const T* __restrict__ stPts,
-                                                   const uint32_t* __restrict__ stRgb, Gridding cubic, double sx,
-                                                   double sy, double sz, uint32_t* __restrict__ lut,
-                                                   OutRecord* __restrict__ out)
+__device__ __forceinline__ void permuteChunk(uint64_t entry, uint32_t N, const uint32_t* __restrict__ stage,
+                                             const uint64_t* __restrict__ dataIdx,
+                                             const uint32_t* __restrict__ pointCount,
+                                             const uint32_t* __restrict__ stLut, const OutRecord* __restrict__ stOut,
+                                             const T* __restrict__ stPts, const uint32_t* __restrict__ stRgb,
+                                             const Gridding& cubic, double sx, double sy, double sz,
+                                             uint32_t* __restrict__ lut, OutRecord* __restrict__ out)
 {
-    const uint64_t entry = chunkList[blockIdx.x];
     const uint32_t id    = static_cast<uint32_t>(entry);
     const uint32_t first = static_cast<uint32_t>(entry >> 32) * PLACE_CHUNK;
     const uint32_t total = pointCount[id];
@@ -1094,6 +1197,21 @@ __global__ __launch_bounds__(BLOCK) void kPermute(const uint64_t* __restrict__ c
         else
             storeRecord(out + dst + j, loadRecord(stOut + s));
     }
+}
+
+template <typename T>
+static __global__ __launch_bounds__(BLOCK) void kPermute(const uint64_t* __restrict__ chunkList, uint32_t N,
+                                                   const uint32_t* __restrict__ stage,
+                                                   const uint64_t* __restrict__ dataIdx,
+                                                   const uint32_t* __restrict__ pointCount,
+                                                   const uint32_t* __restrict__ stLut,
+                                                   const OutRecord* __restrict__ stOut, const T* __restrict__ stPts,
+                                                   const uint32_t* __restrict__ stRgb, Gridding cubic, double sx,
+                                                   double sy, double sz, uint32_t* __restrict__ lut,
+                                                   OutRecord* __restrict__ out)
+{
+    permuteChunk<T>(chunkList[blockIdx.x], N, stage, dataIdx, pointCount, stLut, stOut, stPts, stRgb, cubic, sx, sy,
+                    sz, lut, out);
 }
 
 }  // namespace oc

@@ -10,6 +10,7 @@
 #include <exception>
 #include <string>
 
+#include "batch_builder.h"
 #include "bounds.h"
 #include "log.h"
 #include "reports.h"
@@ -352,5 +353,107 @@ int ocpi_compute_bounding_box_noalloc(const uint8_t* devicePointCloud, uint32_t 
             throw std::runtime_error("bounding box output pointers are null");
         oc::realBoundingBoxNoAlloc(devicePointCloud, pointAmount, dataStride, cloudType, device, realBB);
         oc::cubicBoundingBox(realBB, cubicBB);
+    });
+}
+
+// ---- batched builds -------------------------------------------------------------------------
+
+namespace {
+
+oc::BatchBuilder* BB(void* h)
+{
+    if (!h)
+        throw std::runtime_error("No BatchBuilder is currently initialized!");
+    return static_cast<oc::BatchBuilder*>(h);
+}
+
+}  // namespace
+
+int ocpi_create_batch_builder(void** builder, int device)
+{
+    return execute([&] {
+        if (!builder)
+            throw std::runtime_error("batch builder handle pointer is null");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+            throw std::runtime_error("device " + std::to_string(device) + " does not exist");
+        *builder = new oc::BatchBuilder(device);
+    });
+}
+
+int ocpi_destroy_batch_builder(void* builder)
+{
+    return execute([&] { delete BB(builder); });
+}
+
+int ocpi_batch_build(void* builder, void** sessions, uint32_t count, int32_t* status)
+{
+    return execute([&] {
+        oc::BatchBuilder* b = BB(builder);
+        if (count > 0 && !sessions)
+            throw std::runtime_error("session array is null");
+        std::vector<oc::Session*> list(count);
+        std::vector<oc::BatchRequest> requests(count);
+        for (uint32_t i = 0; i < count; ++i)
+        {
+            list[i] = S(sessions[i]);
+            list[i]->validate();
+            requests[i].cloud       = list[i]->cloudInfo();
+            requests[i].proc        = list[i]->processingInfo();
+            requests[i].hostCloud   = list[i]->hostCloud();
+            requests[i].deviceCloud = list[i]->deviceCloud();
+        }
+        std::vector<oc::BatchOutcome> outcomes = b->build(requests);
+        std::string firstError;
+        for (uint32_t i = 0; i < count; ++i)
+        {
+            const bool ok = static_cast<bool>(outcomes[i].result);
+            if (ok)
+                list[i]->adoptResult(std::move(outcomes[i].result));
+            else if (firstError.empty())
+                firstError = "session " + std::to_string(i) + ": " + outcomes[i].error;
+            if (status)
+                status[i] = ok ? SUCCESS : UNEXPECTED_ERROR;
+        }
+        if (!firstError.empty())
+            throw std::runtime_error(firstError);
+    });
+}
+
+int ocpi_batch_set_launch(void* builder, uint32_t threadGrid, uint32_t tileGrid, uint32_t sortGrid)
+{
+    return execute([&] {
+        if (threadGrid == 0 || tileGrid == 0 || sortGrid == 0 || threadGrid > 65536 || tileGrid > 65536 ||
+            sortGrid > 65536)
+            throw std::runtime_error("launch sizes must be in [1, 65536] workgroups");
+        BB(builder)->setLaunch(oc::BatchLaunch{threadGrid, tileGrid, sortGrid});
+    });
+}
+
+int ocpi_batch_get_timings(void* builder, float* milliseconds, uint32_t n)
+{
+    return execute([&] {
+        const std::vector<float>& t = BB(builder)->timings();
+        for (uint32_t i = 0; i < n && i < t.size(); ++i)
+            milliseconds[i] = t[i];
+    });
+}
+
+int ocpi_estimate_batch_footprint(void** sessions, uint32_t count, uint64_t* bytes)
+{
+    return execute([&] {
+        if (!bytes || (count > 0 && !sessions))
+            throw std::runtime_error("null argument");
+        std::vector<oc::BatchRequest> requests(count);
+        for (uint32_t i = 0; i < count; ++i)
+        {
+            oc::Session* s = S(sessions[i]);
+            s->validate();
+            requests[i].cloud       = s->cloudInfo();
+            requests[i].proc        = s->processingInfo();
+            requests[i].hostCloud   = s->hostCloud();
+            requests[i].deviceCloud = s->deviceCloud();
+        }
+        *bytes = oc::BatchBuilder::estimateFootprint(requests);
     });
 }

@@ -153,7 +153,7 @@ __device__ __forceinline__ void tileRange(uint64_t tiles, uint64_t& first, uint6
 }
 
 template <typename T, typename In>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* blockSums, uint64_t tiles)
+static __global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T* blockSums, uint64_t tiles)
 {
     __shared__ T lds[SCAN_WAVES];
     uint64_t first, last;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanReduce(In in, uint64_t n, T
 
 // exclusive scan of the per-workgroup sums, in place
 template <typename T>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t tiles, T* total)
+static __global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t tiles, T* total)
 {
     __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
     T carry = 0;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void kScanTiles(T* tileSums, uint64_t
 }
 
 template <typename T, typename In, typename Out>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* blockOffsets,
+static __global__ __launch_bounds__(SCAN_THREADS) void kScanDown(In in, Out out, uint64_t n, const T* blockOffsets,
                                                           uint64_t tiles)
 {
     __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
@@ -230,13 +230,13 @@ __device__ __forceinline__ void blockScanAll(In in, Out out, uint64_t n, T* tota
 
 // the same as one launch
 template <typename T, typename In, typename Out>
-__global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
+static __global__ __launch_bounds__(SCAN_THREADS) void kScanSingle(In in, Out out, uint64_t n, T* total)
 {
     __shared__ T lds[SCAN_ITEMS * SCAN_WAVES];
     blockScanAll<T>(in, out, n, total, lds);
 }
 
-inline uint64_t scanTiles(uint64_t n)
+__host__ __device__ inline uint64_t scanTiles(uint64_t n)
 {
     return (n + SCAN_TILE - 1) / SCAN_TILE;
 }

@@ -59,7 +59,16 @@ SIGNATURES = {
     "ocpi_estimate_footprint": [_vp, _vp],
     "ocpi_compute_bounding_box": [_vp, _u32, _u32, _u8, ctypes.c_int, _vp, _vp],
     "ocpi_compute_bounding_box_noalloc": [_vp, _u32, _u32, _u8, ctypes.c_int, _vp, _vp],
+    "ocpi_create_batch_builder": [ctypes.POINTER(_vp), ctypes.c_int],
+    "ocpi_destroy_batch_builder": [_vp],
+    "ocpi_batch_build": [_vp, _vp, _u32, _vp],
+    "ocpi_batch_set_launch": [_vp, _u32, _u32, _u32],
+    "ocpi_batch_get_timings": [_vp, _vp, _u32],
+    "ocpi_estimate_batch_footprint": [_vp, _u32, _vp],
 }
+
+BATCH_ENTRY_POINTS = ["ocpi_create_batch_builder", "ocpi_destroy_batch_builder", "ocpi_batch_build",
+                      "ocpi_batch_set_launch", "ocpi_batch_get_timings", "ocpi_estimate_batch_footprint"]
 
 SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
                         "ocpi_service_set_active_slots", "ocpi_service_get_stats", "ocpi_service_reset_stats",
@@ -68,7 +77,7 @@ SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_ser
 REFERENCE_ENTRY_POINTS = [n for n in SIGNATURES if n not in [
     "ocpi_set_point_cloud_device", "ocpi_set_random_seed", "ocpi_get_last_error", "ocpi_get_build_summary",
     "ocpi_copy_nodes", "ocpi_copy_point_lut", "ocpi_copy_output", "ocpi_get_stage_timings",
-    "ocpi_compute_bounding_box", "ocpi_compute_bounding_box_noalloc"] + SERVICE_ENTRY_POINTS]
+    "ocpi_compute_bounding_box", "ocpi_compute_bounding_box_noalloc"] + SERVICE_ENTRY_POINTS + BATCH_ENTRY_POINTS]
 
 SUMMARY_FIELDS = ("sparse_nodes", "points", "capacity", "used", "internal_nodes", "leaf_nodes", "depth",
                   "dense_nodes", "exported_nodes")
@@ -284,6 +293,63 @@ class Service:
 
     def reset_stats(self):
         _check(self._lib.ocpi_service_reset_stats(self.handle), "ocpi_service_reset_stats")
+
+
+BATCH_TIMING_FIELDS = ("count_ms", "merge_ms", "distribute_ms", "subsample_ms", "device_ms", "host_ms", "copy_ms")
+
+
+class BatchBuilder:
+    """Batched builds (ocpi_create_batch_builder): configured sessions built together, one launch
+    per stage for all clouds of a group; each session then holds its result."""
+
+    def __init__(self, device: int = 0):
+        self._lib = library()
+        h = _vp()
+        _check(self._lib.ocpi_create_batch_builder(ctypes.byref(h), device), "ocpi_create_batch_builder")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _check(self._lib.ocpi_destroy_batch_builder(self.handle), "ocpi_destroy_batch_builder")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, sessions, raise_on_failure: bool = True) -> np.ndarray:
+        """Builds the sessions; returns the per-session status (0 ok, -1 failed)."""
+        handles = (_vp * max(1, len(sessions)))(*[s.handle for s in sessions])
+        status = np.zeros(max(1, len(sessions)), np.int32)
+        rc = self._lib.ocpi_batch_build(self.handle, handles, len(sessions), status.ctypes.data)
+        if rc != 0 and raise_on_failure:
+            raise OctreeError(f"ocpi_batch_build failed: {last_error()}")
+        return status[:len(sessions)]
+
+    def set_launch(self, thread_grid: int, tile_grid: int, sort_grid: int):
+        _check(self._lib.ocpi_batch_set_launch(self.handle, thread_grid, tile_grid, sort_grid),
+               "ocpi_batch_set_launch")
+
+    def timings(self) -> dict:
+        v = np.zeros(len(BATCH_TIMING_FIELDS), np.float32)
+        _check(self._lib.ocpi_batch_get_timings(self.handle, v.ctypes.data, len(v)), "ocpi_batch_get_timings")
+        return {k: float(x) for k, x in zip(BATCH_TIMING_FIELDS, v)}
+
+
+def batch_footprint(sessions) -> int:
+    handles = (_vp * max(1, len(sessions)))(*[s.handle for s in sessions])
+    v = ctypes.c_uint64()
+    _check(library().ocpi_estimate_batch_footprint(handles, len(sessions), ctypes.byref(v)),
+           "ocpi_estimate_batch_footprint")
+    return int(v.value)
 
 
 def set_logging_level(level: int):

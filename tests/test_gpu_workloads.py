@@ -76,6 +76,23 @@ def test_batched_workload_against_oracle():
     _check_all(reqs, sessions, t0, sample_nodes=50)
 
 
+@pytest.mark.timeout(900)
+def test_batched_workload_batch_builder_against_oracle():
+    """The same 256 clouds through the batched pipeline (one launch per stage for all clouds of a
+    record type)."""
+    t0 = time.time()
+    print("case batched workload: 256 clouds through the batch builder", flush=True)
+    reqs = B.batch_requests()
+    sessions = [r.session() for r in reqs]
+    with api.BatchBuilder(0) as bb:
+        status = bb.build(sessions)
+        t = bb.timings()
+    assert (status == 0).all()
+    print(f"  [{time.time() - t0:6.1f} s] built {sum(r.points for r in reqs):,} points: device {t['device_ms']:.1f} ms, "
+          f"host {t['host_ms']:.1f} ms", flush=True)
+    _check_all(reqs, sessions, t0, sample_nodes=50)
+
+
 def serving_set(per_mode=2, rate=100.0, seed=B.SERVE_SEED):
     """The first `per_mode` requests of each mode from the serving generator's seeded stream,
     re-timed to Poisson arrivals at `rate` per second."""
