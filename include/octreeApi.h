@@ -198,9 +198,13 @@ EXPORTED int ocpi_batch_build (void* builder, void** sessions, uint32_t count, i
 /* Persistent workgroups per launch of the batched kernels: those with one item per thread, one
  * subsampling tile per workgroup and one sort chunk per workgroup (each 1..65536). */
 EXPORTED int ocpi_batch_set_launch (void* builder, uint32_t threadGrid, uint32_t tileGrid, uint32_t sortGrid);
+/* 1 (the default): each batch's two launch segments (before and after the host reads the node
+ * counts) run as HIP graphs, captured once per launch shape and replayed; 0: plain launches. */
+EXPORTED int ocpi_batch_use_graphs (void* builder, uint32_t on);
 /* Of the last ocpi_batch_build, summed over its groups: device ms of [0] counting [1] merge
  * [2] distribution [3] subsampling and placement [4] the whole pipeline; host ms of [5] the whole
- * call [6] the result copies. */
+ * call [6] the result copies. With graphs, [0] covers segment 1 (counting and the count pyramid),
+ * [3] segment 2 (node links to placement) and [1], [2] are 0. */
 EXPORTED int ocpi_batch_get_timings (void* builder, float* milliseconds, uint32_t n);
 /* Device bytes a batch of these sessions needs (validates them). */
 EXPORTED int ocpi_estimate_batch_footprint (void** sessions, uint32_t count, uint64_t* bytes);

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -70,6 +72,10 @@ public:
     std::vector<BatchOutcome> build(const std::vector<BatchRequest>& requests);
 
     void setLaunch(const BatchLaunch& launch) { itsLaunch = launch; }
+    // Capture the two launch segments of a batch (before and after the host reads the node
+    // counts) as HIP graphs, cached by everything their launches depend on, and replay them.
+    void useGraphs(bool on) { itsUseGraphs = on; }
+    uint32_t graphsCaptured() const { return itsGraphsCaptured; }
     const BatchLaunch& launch() const { return itsLaunch; }
 
     // Of the last build(), summed over its groups: device ms of [0] count [1] merge [2] distribute
@@ -90,8 +96,14 @@ private:
     template <typename T, uint32_t MODE, bool RANDOM>
     void subsampleGroup(uint32_t B, const BatchLevels& levels, const BatchBounds& bounds);
 
+    void runSegment(bool graphs, const std::vector<uint64_t>& key, const std::function<void()>& issue);
+    void clearGraphs();
+
     int itsDevice;
     hipStream_t itsStream = nullptr;
+    bool itsUseGraphs           = true;
+    uint32_t itsGraphsCaptured  = 0;
+    std::map<std::vector<uint64_t>, hipGraphExec_t> itsGraphs;
     std::shared_ptr<PinnedPool> itsPinned;
     BatchLaunch itsLaunch;
     DeviceBuffer itsArena, itsNodeArena, itsTable, itsPrefix;

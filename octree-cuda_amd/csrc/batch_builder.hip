@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -228,6 +229,50 @@ void carveNodes(Carver& cv, BatchCloud<T>& cc, uint64_t m, const CloudSizes& s)
 
 }  // namespace
 
+void BatchBuilder::runSegment(bool graphs, const std::vector<uint64_t>& key, const std::function<void()>& issue)
+{
+    if (!graphs)
+    {
+        issue();
+        return;
+    }
+    auto it = itsGraphs.find(key);
+    if (it == itsGraphs.end())
+    {
+        // the launches of a segment only depend on the key; every size the device decides and every
+        // address of the batch is read from the cloud table and the prefix arrays at run time
+        OC_HIP_CHECK(hipStreamBeginCapture(itsStream, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            issue();
+        }
+        catch (...)
+        {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(itsStream, &broken);
+            if (broken)
+                (void)hipGraphDestroy(broken);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        OC_HIP_CHECK(hipStreamEndCapture(itsStream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e  = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        OC_HIP_CHECK(e);
+        it = itsGraphs.emplace(key, exec).first;
+        itsGraphsCaptured++;
+    }
+    OC_HIP_CHECK(hipGraphLaunch(it->second, itsStream));
+}
+
+void BatchBuilder::clearGraphs()
+{
+    for (auto& kv : itsGraphs)
+        (void)hipGraphExecDestroy(kv.second);
+    itsGraphs.clear();
+}
+
 BatchBuilder::BatchBuilder(int device, std::shared_ptr<PinnedPool> pinned) : itsDevice(device), itsPinned(pinned)
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));
@@ -240,6 +285,7 @@ BatchBuilder::BatchBuilder(int device, std::shared_ptr<PinnedPool> pinned) : its
 
 BatchBuilder::~BatchBuilder()
 {
+    clearGraphs();
     if (itsStream)
     {
         (void)hipStreamSynchronize(itsStream);
@@ -253,6 +299,7 @@ BatchBuilder::~BatchBuilder()
 void BatchBuilder::releaseArena()
 {
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+    clearGraphs();  // they hold the addresses of the table and the prefix arrays
     itsArena.release();
     itsNodeArena.release();
     itsTable.release();
@@ -424,8 +471,29 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     const BatchCloud<T>* ctx = itsTable.as<BatchCloud<T>>();
     const BatchLaunch& LG    = itsLaunch;
 
-    // ---- count ---------------------------------------------------------------------------------
-    OC_HIP_CHECK(hipEventRecord(itsEvents[0], s));
+    bool split = false;
+    for (const CloudSizes& sz : sizes)
+        split |= sz.n > FINE_SPLIT;
+    FineSortParams fp{};
+    fp.lc         = L.lc;
+    fp.bucketGrid = first.proc.chunkingGrid >> L.lc;
+    for (uint32_t l = 0; l <= L.depth; ++l)
+    {
+        fp.levelOffset[l] = L.offset[l];
+        fp.levelGrid[l]   = L.grid[l];
+    }
+    LevelOffsets lo{};
+    for (uint32_t l = 0; l <= L.depth; ++l)
+        lo.offset[l] = L.offset[l];
+    lo.levels = L.depth + 1;
+    const bool graphs = itsUseGraphs;
+    auto mark         = [&](int k) {
+        if (!graphs)
+            OC_HIP_CHECK(hipEventRecord(itsEvents[k], s));
+    };
+
+    // ---- segment 1: count, pyramid, canonical ids, level starts ---------------------------------
+    auto segment1 = [&]() {
     kBatchInit<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B0, L.buckets);
     OC_LAUNCH_CHECK();
     const uint32_t fineBits = 3 * L.lc;
@@ -435,19 +503,8 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     OC_LAUNCH_CHECK();
     kBucketScatterB<T><<<LG.sortGrid, SORT_THREADS, 0, s>>>(ctx, B0, slot(P_SCATTER), fineBits);
     OC_LAUNCH_CHECK();
-    FineSortParams fp{};
-    fp.lc         = L.lc;
-    fp.bucketGrid = first.proc.chunkingGrid >> L.lc;
-    for (uint32_t l = 0; l <= L.depth; ++l)
-    {
-        fp.levelOffset[l] = L.offset[l];
-        fp.levelGrid[l]   = L.grid[l];
-    }
     kFineSortB<T><<<LG.sortGrid, SORT_THREADS, 0, s>>>(ctx, B0, fp, L.buckets);
     OC_LAUNCH_CHECK();
-    bool split = false;
-    for (const CloudSizes& sz : sizes)
-        split |= sz.n > FINE_SPLIT;
     if (split)
     {
         kSplitChunksB<T><<<std::min<uint32_t>(B0, 1024), SCAN_THREADS, 0, s>>>(ctx, B0, L.buckets);
@@ -457,9 +514,7 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
         kFineChunkPermB<T><<<LG.sortGrid, SORT_THREADS, 0, s>>>(ctx, B0, slot(P_SPLIT), L.lc);
         OC_LAUNCH_CHECK();
     }
-    OC_HIP_CHECK(hipEventRecord(itsEvents[1], s));
-
-    // ---- merge: pyramid, canonical ids, level starts; the one host wait before the nodes ------
+    mark(1);
     for (uint32_t l = L.lc + 1; l <= L.depth; ++l)
     {
         kPropagateB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B0, L.offset[l], L.grid[l], L.offset[l - 1],
@@ -468,12 +523,16 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     }
     scanBatch<uint32_t>(OccupancyJob<T>{ctx, L.dense}, B0, L.dense, s);
     OC_LAUNCH_CHECK();
-    LevelOffsets lo{};
-    for (uint32_t l = 0; l <= L.depth; ++l)
-        lo.offset[l] = L.offset[l];
-    lo.levels = L.depth + 1;
     kLevelStartsB<T><<<B0, 64, 0, s>>>(ctx, lo);
     OC_LAUNCH_CHECK();
+    };
+    OC_HIP_CHECK(hipEventRecord(itsEvents[0], s));
+    runSegment(graphs, {1, sizeof(T), first.proc.chunkingGrid, B0, split ? 1u : 0u, LG.threadGrid, LG.tileGrid,
+                        LG.sortGrid},
+               segment1);
+    if (graphs)
+        OC_HIP_CHECK(hipEventRecord(itsEvents[1], s));
+    // the one host wait before the nodes
     itsHostScalars.resize(static_cast<size_t>(B0) * BS_SLOTS);
     OC_HIP_CHECK(hipMemcpyAsync(itsHostScalars.data(), scalarsBase, sizeof(uint64_t) * itsHostScalars.size(),
                                 hipMemcpyDeviceToHost, s));
@@ -538,14 +597,15 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
         hostPrefix(P_LEVEL + l, B, [&](uint32_t j) { return lsOf(j)[l + 1] - lsOf(j)[l]; });
     uploadTables(B);
 
-    // ---- merge: nodes, flags, leaf blocks, cell -> leaf ------------------------------------------
+    // ---- segment 2: nodes, distribution, subsampling, placement ---------------------------------
+    auto segment2 = [&]() {
     kNodeLinksB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_NODES), L);
     OC_LAUNCH_CHECK();
     scanBatch<uint32_t>(LeafBlockJob<T>{ctx}, B, maxM, s);
     kLeafDataIdxB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_NODES));
     kCellLeafB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_LEVEL0));
     OC_LAUNCH_CHECK();
-    OC_HIP_CHECK(hipEventRecord(itsEvents[2], s));
+    mark(2);
 
     // ---- distribution: leaf and cell order of the sorted clouds, then the leaf blocks ----------
     kSortedStartB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_NODES), L);
@@ -555,7 +615,7 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     kGatherSortedB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_POINTS));
     kDistributeSortedB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_DIST));
     OC_LAUNCH_CHECK();
-    OC_HIP_CHECK(hipEventRecord(itsEvents[3], s));
+    mark(3);
 
     // ---- subsampling, level by level for all clouds --------------------------------------------
     switch (mode)
@@ -583,19 +643,40 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     kWorkPrefix<<<1, SCAN_THREADS, 0, s>>>(ScalarCount<T>{ctx, BS_TABLE_TOTAL, true}, B, slot(P_PLACE));
     kPermuteB<T><<<LG.threadGrid, BLOCK, 0, s>>>(ctx, B, slot(P_PLACE));
     OC_LAUNCH_CHECK();
+    };
+    if (graphs)
+        OC_HIP_CHECK(hipEventRecord(itsEvents[2], s));
+    runSegment(graphs,
+               {2, sizeof(T), first.proc.chunkingGrid, B, static_cast<uint64_t>(mode), random ? 1u : 0u,
+                LG.threadGrid, LG.tileGrid, LG.sortGrid, scanWidth(maxM), scanWidth(bounds.segments),
+                scanWidth(bounds.words), scanWidth(bounds.cells), scanWidth(bounds.levelNodes[1]),
+                scanWidth(bounds.levelNodes[2]), scanWidth(bounds.levelNodes[3]), scanWidth(bounds.levelNodes[4]),
+                scanWidth(bounds.levelNodes[5]), scanWidth(bounds.levelNodes[6]), scanWidth(bounds.levelNodes[7]),
+                scanWidth(bounds.levelNodes[8]), scanWidth(bounds.levelNodes[9])},
+               segment2);
     OC_HIP_CHECK(hipEventRecord(itsEvents[4], s));
 
     // ---- results -------------------------------------------------------------------------------
-    for (uint32_t j = 0; j < B; ++j)  // the live clouds' scalars, one copy each (they are not adjacent)
-        OC_HIP_CHECK(hipMemcpyAsync(itsHostScalars.data() + static_cast<uint64_t>(live[j]) * BS_SLOTS, table[j].sc,
-                                    sizeof(uint64_t) * BS_SLOTS, hipMemcpyDeviceToHost, s));
+    OC_HIP_CHECK(hipMemcpyAsync(itsHostScalars.data(), scalarsBase, sizeof(uint64_t) * itsHostScalars.size(),
+                                hipMemcpyDeviceToHost, s));
     OC_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
-    const int stageEv[5] = {0, 1, 2, 3, 4};
-    for (int k = 0; k < 4; ++k)
+    if (graphs)
     {
-        OC_HIP_CHECK(hipEventElapsedTime(&ms, itsEvents[stageEv[k]], itsEvents[stageEv[k + 1]]));
-        itsTimings[k] += ms;
+        // segment 1 (count and pyramid) and segment 2 (nodes to placement)
+        OC_HIP_CHECK(hipEventElapsedTime(&ms, itsEvents[0], itsEvents[1]));
+        itsTimings[0] += ms;
+        OC_HIP_CHECK(hipEventElapsedTime(&ms, itsEvents[2], itsEvents[4]));
+        itsTimings[3] += ms;
+    }
+    else
+    {
+        const int stageEv[5] = {0, 1, 2, 3, 4};
+        for (int k = 0; k < 4; ++k)
+        {
+            OC_HIP_CHECK(hipEventElapsedTime(&ms, itsEvents[stageEv[k]], itsEvents[stageEv[k + 1]]));
+            itsTimings[k] += ms;
+        }
     }
     OC_HIP_CHECK(hipEventElapsedTime(&ms, itsEvents[0], itsEvents[4]));
     itsTimings[4] += ms;

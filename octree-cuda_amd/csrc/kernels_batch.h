@@ -283,11 +283,16 @@ static __global__ __launch_bounds__(SCAN_THREADS) void kScanDownB(Job job)
 
 // maxItems: an upper bound of any cloud's item count; it sizes the workgroups per cloud (each at
 // least 8 tiles of the largest cloud), so small scans do not launch mostly idle workgroups.
+inline uint32_t scanWidth(uint64_t maxItems)
+{
+    const uint64_t tiles = scanTiles(maxItems);
+    return static_cast<uint32_t>(std::min<uint64_t>(BATCH_SCAN_X, std::max<uint64_t>(1, (tiles + 7) / 8)));
+}
+
 template <typename V, typename Job>
 void scanBatch(Job job, uint32_t B, uint64_t maxItems, hipStream_t stream)
 {
-    const uint64_t tiles = scanTiles(maxItems);
-    const uint32_t X     = static_cast<uint32_t>(std::min<uint64_t>(BATCH_SCAN_X, std::max<uint64_t>(1, (tiles + 7) / 8)));
+    const uint32_t X = scanWidth(maxItems);
     const dim3 grid(X, B);
     kScanReduceB<V, Job><<<grid, SCAN_THREADS, 0, stream>>>(job);
     kScanTilesB<V, Job><<<dim3(1, B), SCAN_THREADS, 0, stream>>>(job, X);

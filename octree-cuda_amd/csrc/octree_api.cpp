@@ -430,6 +430,11 @@ int ocpi_batch_set_launch(void* builder, uint32_t threadGrid, uint32_t tileGrid,
     });
 }
 
+int ocpi_batch_use_graphs(void* builder, uint32_t on)
+{
+    return execute([&] { BB(builder)->useGraphs(on != 0); });
+}
+
 int ocpi_batch_get_timings(void* builder, float* milliseconds, uint32_t n)
 {
     return execute([&] {

@@ -63,12 +63,14 @@ SIGNATURES = {
     "ocpi_destroy_batch_builder": [_vp],
     "ocpi_batch_build": [_vp, _vp, _u32, _vp],
     "ocpi_batch_set_launch": [_vp, _u32, _u32, _u32],
+    "ocpi_batch_use_graphs": [_vp, _u32],
     "ocpi_batch_get_timings": [_vp, _vp, _u32],
     "ocpi_estimate_batch_footprint": [_vp, _u32, _vp],
 }
 
 BATCH_ENTRY_POINTS = ["ocpi_create_batch_builder", "ocpi_destroy_batch_builder", "ocpi_batch_build",
-                      "ocpi_batch_set_launch", "ocpi_batch_get_timings", "ocpi_estimate_batch_footprint"]
+                      "ocpi_batch_set_launch", "ocpi_batch_use_graphs", "ocpi_batch_get_timings",
+                      "ocpi_estimate_batch_footprint"]
 
 SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
                         "ocpi_service_set_active_slots", "ocpi_service_get_stats", "ocpi_service_reset_stats",
@@ -337,6 +339,9 @@ class BatchBuilder:
     def set_launch(self, thread_grid: int, tile_grid: int, sort_grid: int):
         _check(self._lib.ocpi_batch_set_launch(self.handle, thread_grid, tile_grid, sort_grid),
                "ocpi_batch_set_launch")
+
+    def use_graphs(self, on: bool):
+        _check(self._lib.ocpi_batch_use_graphs(self.handle, 1 if on else 0), "ocpi_batch_use_graphs")
 
     def timings(self) -> dict:
         v = np.zeros(len(BATCH_TIMING_FIELDS), np.float32)

@@ -53,9 +53,11 @@ class Request:
     def stride(self):
         return 15 if self.cloud_type == 0 else 27
 
-    def session(self, device: int = 0) -> api.Session:
+    def session(self, device: int = 0, device_pointer: int | None = None) -> api.Session:
+        """A configured session; with device_pointer, the cloud is read in place from HBM."""
         s = api.Session(device)
-        s.set_cloud(self.cloud, self.points, self.stride, self.cloud_type, self.cubic, [self.scale] * 3)
+        s.set_cloud(self.cloud, self.points, self.stride, self.cloud_type, self.cubic, [self.scale] * 3,
+                    device_pointer=device_pointer)
         s.configure(self.build)
         return s
 
@@ -143,6 +145,70 @@ def tune_slots(service: api.Service, requests, candidates=(1, 2, 3, 4, 6, 8), sa
         results[k] = max(run_batch(service, subset, k)["points_per_s"] for _ in range(repeats))
     best = max(results, key=results.get)
     return best, results
+
+
+def upload_clouds(requests):
+    """All request clouds in one device array (256-byte aligned); returns (array, addresses)."""
+    from .device import DeviceArray
+    offsets, total = [], 0
+    for r in requests:
+        offsets.append(total)
+        total += (r.points * r.stride + 255) & ~255
+    arr = DeviceArray(total)
+    for r, off in zip(requests, offsets):
+        arr.upload(r.cloud[: r.points * r.stride], off)
+    return arr, [arr.address(off) for off in offsets]
+
+
+def run_batch_builder(builder: api.BatchBuilder, requests, addresses=None):
+    """All requests built in one batched call (ocpi_batch_build); with addresses (clouds already in
+    HBM, upload_clouds) the clouds are read in place, otherwise uploaded from host memory. Every
+    request is submitted at t0 and finishes when the call returns, so p50 = p99 = the call's time.
+    Returns throughput figures: end to end (results in host memory) and the device pipeline alone."""
+    sessions = [r.session(device_pointer=None if addresses is None else addresses[i])
+                for i, r in enumerate(requests)]
+    t0 = time.perf_counter()
+    status = builder.build(sessions, raise_on_failure=False)
+    wall = time.perf_counter() - t0
+    t = builder.timings()
+    for s in sessions:
+        s.close()
+    points = sum(r.points for r in requests)
+    return {"clouds": len(requests), "points": points, "seconds": wall, "points_per_s": points / wall,
+            "device_ms": t["device_ms"], "device_points_per_s": points / (t["device_ms"] * 1e-3),
+            "copy_ms": t["copy_ms"], "p50_ms": wall * 1e3, "p99_ms": wall * 1e3,
+            "failed": int(np.count_nonzero(status))}
+
+
+def tune_batch_launch(builder: api.BatchBuilder, requests, addresses=None, repeats: int = 2,
+                      candidates=None):
+    """Launch-shape autotuner of the batched pipeline: coordinate descent over the persistent
+    workgroups per launch of its three kernel classes (sort chunks of 1024 threads, subsampling
+    tiles, one item per thread), each shape timed by the device time of the whole batch, best of
+    `repeats` builds (the first build of a shape also captures its graphs). Returns the chosen
+    (thread, tile, sort) grid and every measurement."""
+    if candidates is None:
+        candidates = {"sort": (256, 512, 1024, 2048), "tile": (1024, 2048, 4096, 8192, 16384),
+                      "thread": (512, 1024, 2048, 4096, 8192)}
+    best = {"thread": 2048, "tile": 4096, "sort": 1024}
+    history = []
+
+    def measure(shape):
+        builder.set_launch(shape["thread"], shape["tile"], shape["sort"])
+        times = []
+        for _ in range(repeats):
+            r = run_batch_builder(builder, requests, addresses)
+            if r["failed"]:
+                raise RuntimeError(f"{r['failed']} requests failed while tuning")
+            times.append(r["device_ms"])
+        history.append({**shape, "device_ms": min(times)})
+        return min(times)
+
+    for knob in ("sort", "tile", "thread"):
+        results = {c: measure({**best, knob: c}) for c in candidates[knob]}
+        best[knob] = min(results, key=results.get)
+    builder.set_launch(best["thread"], best["tile"], best["sort"])
+    return best, history
 
 
 PREFIX_BLOCK = 1 << 16

@@ -104,13 +104,18 @@ def test_batch_matches_the_oracle():
         s.close()
 
 
-def test_batch_rebuild_with_other_launch_shapes():
-    """The same batch twice on one builder (arena reused), the second time with small persistent
-    grids so that every workgroup loops over many items: the results are the oracle's both times."""
+def test_batch_graph_replays_and_launch_shapes():
+    """One builder (arena reused) builds the same batch six times: with plain launches, then three
+    times through its captured graphs (the first build captures, the next two replay, with the
+    host's table uploads, readbacks and checks between the replays), then with small persistent
+    grids (every workgroup loops over many items) captured and plain. Every build equals the
+    oracle's result."""
     reqs = [mixed_requests()[k] for k in (0, 3, 5, 9, 16, 19)]
     with api.BatchBuilder(0) as bb:
-        for shape in ((2048, 4096, 1024), (7, 5, 3)):
-            print(f"case batch rebuild, launch shape {shape}", flush=True)
+        for graphs, shape in ((False, (2048, 4096, 1024)), (True, (2048, 4096, 1024)), (True, (2048, 4096, 1024)),
+                              (True, (2048, 4096, 1024)), (True, (7, 5, 3)), (False, (7, 5, 3))):
+            print(f"case batch rebuild, graphs {graphs}, launch shape {shape}", flush=True)
+            bb.use_graphs(graphs)
             bb.set_launch(*shape)
             sessions = [_session(r) for r in reqs]
             status = bb.build(sessions)
