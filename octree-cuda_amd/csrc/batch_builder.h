@@ -84,7 +84,10 @@ public:
     const std::vector<float>& timings() const { return itsTimings; }
     uint32_t groupsBuilt() const { return itsGroups; }
 
-    uint64_t arenaBytes() const { return itsArena.capacity() + itsNodeArena.capacity(); }
+    uint64_t arenaBytes() const
+    {
+        return itsArena.capacity() + itsNodeArena.capacity() + itsPacked.capacity() + itsPieces.capacity();
+    }
     void releaseArena();
     // Device bytes a batch of these requests needs (the arena sizes build() allocates).
     static uint64_t estimateFootprint(const std::vector<BatchRequest>& requests);
@@ -106,7 +109,7 @@ private:
     std::map<std::vector<uint64_t>, hipGraphExec_t> itsGraphs;
     std::shared_ptr<PinnedPool> itsPinned;
     BatchLaunch itsLaunch;
-    DeviceBuffer itsArena, itsNodeArena, itsTable, itsPrefix;
+    DeviceBuffer itsArena, itsNodeArena, itsTable, itsPrefix, itsPacked, itsPieces;
     std::vector<uint8_t> itsHostTable;
     std::vector<uint32_t> itsHostPrefix;
     std::vector<uint64_t> itsHostScalars;

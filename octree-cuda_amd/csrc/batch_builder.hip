@@ -302,6 +302,8 @@ void BatchBuilder::releaseArena()
     clearGraphs();  // they hold the addresses of the table and the prefix arrays
     itsArena.release();
     itsNodeArena.release();
+    itsPacked.release();
+    itsPieces.release();
     itsTable.release();
     itsPrefix.release();
 }
@@ -702,30 +704,40 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     }
     if (total == 0)
         return;
-    std::shared_ptr<uint8_t> block = itsPinned->acquire(total);
+    // the snapshots are packed on the device in their host layout and come home in one copy
+    std::vector<PackPiece> pieces;
     for (uint32_t j = 0; j < B; ++j)
     {
-        BatchOutcome& o = outcomes[members[live[j]]];
-        if (!o.error.empty())
+        if (!outcomes[members[live[j]]].error.empty())
             continue;
         const BatchCloud<T>& cc = table[j];
         const uint64_t m        = cc.m;
         const SnapshotLayout l(cc.m, used[j]);
-        uint8_t* dst = block.get() + offsets[j];
-        auto copy    = [&](uint64_t offset, const void* src, uint64_t bytes) {
-            if (bytes)
-                OC_HIP_CHECK(hipMemcpyAsync(dst + offset, src, bytes, hipMemcpyDeviceToHost, s));
+        auto piece = [&](uint64_t offset, const void* src, uint64_t bytes) {
+            for (uint64_t at = 0; at < bytes; at += PACK_CHUNK)
+                pieces.push_back(PackPiece{static_cast<const uint8_t*>(src) + at, offsets[j] + offset + at,
+                                           std::min<uint64_t>(PACK_CHUNK, bytes - at)});
         };
-        copy(l.dataIdx, cc.nodeData, 8 * m);
-        copy(l.denseKey, cc.keyOut, 4 * m);
-        copy(l.count, cc.countOut, 4 * m);
-        copy(l.pointCount, cc.nodePoints, 4 * m);
-        copy(l.parent, cc.nodeParent, 4 * m);
-        copy(l.children, cc.nodeChild, 32 * m);
-        copy(l.flags, cc.nodeFlags, m);
-        copy(l.lut, cc.lut, 4 * used[j]);
-        copy(l.out, cc.out, sizeof(OutRecord) * used[j]);
+        piece(l.dataIdx, cc.nodeData, 8 * m);
+        piece(l.denseKey, cc.keyOut, 4 * m);
+        piece(l.count, cc.countOut, 4 * m);
+        piece(l.pointCount, cc.nodePoints, 4 * m);
+        piece(l.parent, cc.nodeParent, 4 * m);
+        piece(l.children, cc.nodeChild, 32 * m);
+        piece(l.flags, cc.nodeFlags, m);
+        piece(l.lut, cc.lut, 4 * used[j]);
+        piece(l.out, cc.out, sizeof(OutRecord) * used[j]);
     }
+    itsPacked.ensure(total);
+    itsPieces.ensure(sizeof(PackPiece) * pieces.size());
+    OC_HIP_CHECK(hipMemcpyAsync(itsPieces.get(), pieces.data(), sizeof(PackPiece) * pieces.size(),
+                                hipMemcpyHostToDevice, s));
+    kPackResults<<<std::min<size_t>(pieces.size(), 8192), BLOCK, 0, s>>>(itsPieces.as<PackPiece>(),
+                                                                          static_cast<uint32_t>(pieces.size()),
+                                                                          itsPacked.as<uint8_t>());
+    OC_LAUNCH_CHECK();
+    std::shared_ptr<uint8_t> block = itsPinned->acquire(total);
+    OC_HIP_CHECK(hipMemcpyAsync(block.get(), itsPacked.get(), total, hipMemcpyDeviceToHost, s));
     OC_HIP_CHECK(hipStreamSynchronize(s));
     for (uint32_t j = 0; j < B; ++j)
     {

@@ -70,22 +70,23 @@ public:
     uint64_t snapshotBytes() const;
 };
 
-// Packing of a snapshot: the node arrays (dataIdx first, for its 8-byte alignment), then the used
-// LUT entries and output records.
+// Packing of a snapshot: the node arrays, then the used LUT entries and output records, each
+// section starting on a 16-byte boundary (so a device-side packer copies whole 16-byte words).
 struct SnapshotLayout
 {
     uint64_t dataIdx, denseKey, count, pointCount, parent, children, flags, lut, out, bytes;
     SnapshotLayout(uint32_t m, uint64_t used)
     {
+        auto next  = [](uint64_t at, uint64_t size) { return (at + size + 15) & ~15ull; };
         dataIdx    = 0;
-        denseKey   = dataIdx + 8ull * m;
-        count      = denseKey + 4ull * m;
-        pointCount = count + 4ull * m;
-        parent     = pointCount + 4ull * m;
-        children   = parent + 4ull * m;
-        flags      = children + 32ull * m;
-        lut        = (flags + m + 3) & ~3ull;
-        out        = lut + 4 * used;
+        denseKey   = next(dataIdx, 8ull * m);
+        count      = next(denseKey, 4ull * m);
+        pointCount = next(count, 4ull * m);
+        parent     = next(pointCount, 4ull * m);
+        children   = next(parent, 4ull * m);
+        flags      = next(children, 32ull * m);
+        lut        = next(flags, m);
+        out        = next(lut, 4 * used);
         bytes      = out + 18 * used;
     }
 };

@@ -1104,4 +1104,34 @@ static __global__ __launch_bounds__(BLOCK) void kPermuteB(const BatchCloud<T>* _
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Results: every cloud's snapshot packed into one device buffer, so one copy brings them home
+// ------------------------------------------------------------------------------------------------
+
+constexpr uint64_t PACK_CHUNK = 256 * 1024;  // bytes per work item of the packer
+
+// A piece of at most PACK_CHUNK bytes: src and dst are 16-byte aligned.
+struct PackPiece
+{
+    const uint8_t* src;
+    uint64_t dst;
+    uint64_t bytes;
+};
+
+static __global__ __launch_bounds__(BLOCK) void kPackResults(const PackPiece* __restrict__ pieces, uint32_t count,
+                                                              uint8_t* __restrict__ dst)
+{
+    for (uint32_t p = blockIdx.x; p < count; p += gridDim.x)
+    {
+        const PackPiece piece = pieces[p];
+        const uint64_t words  = piece.bytes / 16;
+        const uint4* s        = reinterpret_cast<const uint4*>(piece.src);
+        uint4* d              = reinterpret_cast<uint4*>(dst + piece.dst);
+        for (uint64_t w = threadIdx.x; w < words; w += BLOCK)
+            d[w] = s[w];
+        for (uint64_t b = words * 16 + threadIdx.x; b < piece.bytes; b += BLOCK)
+            dst[piece.dst + b] = piece.src[b];
+    }
+}
+
 }  // namespace oc
