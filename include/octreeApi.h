@@ -144,6 +144,14 @@ EXPORTED int ocpi_get_stage_timings (void* session, float* milliseconds, uint32_
  * copies then work as after ocpi_generate_octree. The session and its cloud buffer must stay
  * untouched until ocpi_service_wait has returned for it. */
 EXPORTED int ocpi_create_service (void** service, int device, uint32_t slots, uint64_t memoryBudget);
+/* A service that builds in batches (continuous batching over the batched pipeline): `workers`
+ * workers, each with a batch builder; a free worker takes every queued request that fits the
+ * memory budget next to the other workers' arenas, up to maxBatch, and builds them together (one
+ * launch per stage for all of them); requests arriving meanwhile wait for the next batch, which
+ * another worker may build while the first batch is still on the GPU. Submit, wait, stats and
+ * destroy work as for ocpi_create_service; set_active_slots sets the active workers. */
+EXPORTED int ocpi_create_batch_service (
+        void** service, int device, uint32_t workers, uint32_t maxBatch, uint64_t memoryBudget);
 /* Fails requests still queued and waits for the running ones. */
 EXPORTED int ocpi_destroy_service (void* service);
 /* Validates the session's configuration and queues it (-1 if invalid or above the budget). */
@@ -156,8 +164,9 @@ EXPORTED int ocpi_service_set_active_slots (void* service, uint32_t slots);
  * [6] p50 latency ms [7] p99 latency ms [8] mean latency ms [9] reserved device bytes
  * [10] memory budget [11] active slots [12] slots [13] mean queue wait ms [14] max latency ms
  * [15] peak reserved device bytes [16] [17] [18] mean time per completed request in setup (arena
- * sizing and cloud upload), generate and result capture, ms. Latency = submit to result attached
- * to the session. */
+ * sizing and cloud upload), generate and result capture, ms (batch service: [17] is the time of
+ * the request's batch) [19] batches built [20] mean requests per batch. Latency = submit to result
+ * attached to the session. */
 EXPORTED int ocpi_service_get_stats (void* service, double* stats, uint32_t n);
 EXPORTED int ocpi_service_reset_stats (void* service);
 /* Of the session's last service request: [0] queue wait ms [1] build + copy ms [2] total ms. */

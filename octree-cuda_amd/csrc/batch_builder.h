@@ -24,22 +24,10 @@
 #include "build_result.h"
 #include "builder.h"
 #include "pinned_pool.h"
+#include "slot_builder.h"
 
 namespace oc {
 
-struct BatchRequest
-{
-    CloudInfo cloud;
-    ProcessingInfo proc;
-    const uint8_t* hostCloud   = nullptr;  // borrowed until build() returns
-    const uint8_t* deviceCloud = nullptr;
-};
-
-struct BatchOutcome
-{
-    std::unique_ptr<HostBuildResult> result;  // null when the build failed
-    std::string error;
-};
 
 // Launch shapes of the batched kernels, the autotuner's knobs: persistent workgroups per launch
 // for the kernels with one item per thread (256 threads), one tile per workgroup (256 threads)
@@ -60,7 +48,7 @@ struct BatchBounds
 
 struct BatchLevels;
 
-class BatchBuilder
+class BatchBuilder final : public BatchSlotBuilder
 {
 public:
     explicit BatchBuilder(int device, std::shared_ptr<PinnedPool> pinned = nullptr);
@@ -69,7 +57,7 @@ public:
     BatchBuilder& operator=(const BatchBuilder&) = delete;
 
     // Builds every request (validated by the caller) and returns one outcome per request, in order.
-    std::vector<BatchOutcome> build(const std::vector<BatchRequest>& requests);
+    std::vector<BatchOutcome> build(const std::vector<BatchRequest>& requests) override;
 
     void setLaunch(const BatchLaunch& launch) { itsLaunch = launch; }
     // Capture the two launch segments of a batch (before and after the host reads the node
@@ -84,11 +72,11 @@ public:
     const std::vector<float>& timings() const { return itsTimings; }
     uint32_t groupsBuilt() const { return itsGroups; }
 
-    uint64_t arenaBytes() const
+    uint64_t arenaBytes() const override
     {
         return itsArena.capacity() + itsNodeArena.capacity() + itsPacked.capacity() + itsPieces.capacity();
     }
-    void releaseArena();
+    void releaseArena() override;
     // Device bytes a batch of these requests needs (the arena sizes build() allocates).
     static uint64_t estimateFootprint(const std::vector<BatchRequest>& requests);
 

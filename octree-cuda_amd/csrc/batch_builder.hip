@@ -273,6 +273,16 @@ void BatchBuilder::clearGraphs()
     itsGraphs.clear();
 }
 
+std::unique_ptr<BatchSlotBuilder> makeBatchSlotBuilder(int device, std::shared_ptr<PinnedPool> pinned)
+{
+    return std::make_unique<BatchBuilder>(device, std::move(pinned));
+}
+
+uint64_t batchFootprint(const std::vector<BatchRequest>& requests)
+{
+    return BatchBuilder::estimateFootprint(requests);
+}
+
 BatchBuilder::BatchBuilder(int device, std::shared_ptr<PinnedPool> pinned) : itsDevice(device), itsPinned(pinned)
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));

@@ -20,11 +20,13 @@
 
 #include <cstdint>
 #include <array>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "build_result.h"
+#include "slot_builder.h"
 #include "types.h"
 
 namespace oc {
@@ -57,7 +59,7 @@ private:
     size_t itsBytes = 0;
 };
 
-class OctreeBuilder final : public BuildResult
+class OctreeBuilder final : public BuildResult, public SlotBuilder
 {
 public:
     // A builder owns one HIP stream and a grow-only device arena; setup() prepares it for one
@@ -71,15 +73,17 @@ public:
     OctreeBuilder& operator=(const OctreeBuilder&) = delete;
 
     void setup(const CloudInfo& cloud, const ProcessingInfo& proc, const uint8_t* hostCloud,
-               const uint8_t* deviceCloud);
-    void generate();
+               const uint8_t* deviceCloud) override;
+    void generate() override;
+    const BuildResult& lastBuild() const override { return *this; }
+    void setProgress(std::function<void()> hook) override { itsProgress = std::move(hook); }
     // Per-batch subsampling events (the insert / emit split of the stage timings); off, a build
     // makes fewer runtime calls, which matters when many builders share one process.
     void setDetailedTimings(bool on) { itsDetailedTimings = on; }
 
     // Device bytes held by the arena, and its release (the next setup grows it again).
-    uint64_t arenaBytes() const;
-    void releaseArena();
+    uint64_t arenaBytes() const override;
+    void releaseArena() override;
     // Upper bound of the arena a build of this cloud and configuration needs, from the same sizes
     // setup() and the build allocate (the output capacity E = N * outputFactor drives most terms).
     static uint64_t estimateFootprint(const CloudInfo& cloud, const ProcessingInfo& proc, bool hostCloud);
@@ -103,8 +107,10 @@ public:
     // The snapshot copies on the builder's copy stream, after the build: the host goes on (the
     // next setup's upload overlaps them); the next setup / generate makes the build stream wait
     // for them before it overwrites anything they read. waitSnapshot blocks until they are done.
-    void snapshotAsync(uint8_t* dst);
-    void waitSnapshot();
+    void snapshotAsync(uint8_t* dst) override;
+    void waitSnapshot() override;
+    bool snapshotDone() override;
+    void drainCopies() noexcept override;
 
 private:
     template <typename T>
@@ -122,6 +128,7 @@ private:
     void placeLevelByLevel();
 
     void syncStream();
+    void progress();
     void waitForSnapshotOnStream();
     template <typename V>
     V readDevice(const V* devicePtr);
@@ -141,6 +148,7 @@ private:
     uint64_t itsMailSeq   = 0;
     bool itsReady         = false;  // setup() has run
     bool itsDetailedTimings = true;
+    std::function<void()> itsProgress;  // called between the stages of generate()
     CloudInfo itsCloud;
     ProcessingInfo itsProc;
     ColorMode itsColorMode = COLOR_NONE;

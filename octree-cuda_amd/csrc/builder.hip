@@ -192,6 +192,15 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     itsCount.ensure(sizeof(uint32_t) * static_cast<uint64_t>(itsDense));
     itsDenseToSparse.ensure(sizeof(int32_t) * static_cast<uint64_t>(itsDense));
     uint64_t mBound = std::min<uint64_t>(itsDense, static_cast<uint64_t>(itsDepth + 1) * cloud.pointAmount);
+    // The previous build's snapshot copies may still read the node keys and counts, the LUT and
+    // the output: growing one of them frees it, so those copies are waited for first. (The other
+    // arrays a snapshot reads are grown by the merge, after its stream has waited for the copies
+    // and the host has read the node count back.)
+    if (itsSnapshotPending && (sizeof(uint32_t) * mBound > itsNodeKey.capacity() ||
+                               sizeof(uint32_t) * mBound > itsNodeCount.capacity() ||
+                               sizeof(uint32_t) * itsE > itsLut.capacity() ||
+                               sizeof(OutRecord) * itsE > itsOut.capacity()))
+        waitSnapshot();
     itsNodeKey.ensure(sizeof(uint32_t) * mBound);
     itsNodeCount.ensure(sizeof(uint32_t) * mBound);
     itsLut.ensure(sizeof(uint32_t) * itsE);
@@ -419,10 +428,13 @@ void OctreeBuilder::generateTyped()
 
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_START], itsStream));
     countPoints<T>();  // writes nothing a snapshot reads, so it overlaps the previous one's copies
+    progress();
     waitForSnapshotOnStream();
     mergeLevels();
+    progress();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_MERGE_END], itsStream));
     distributePoints<T>();
+    progress();
     subsampleLevels<T>();
     placeInternalNodes();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_END], itsStream));
@@ -995,6 +1007,7 @@ void OctreeBuilder::subsampleLevels()
                 OC_HIP_CHECK(hipEventRecord(ev[3], itsStream));
             cursor += cells;
         }
+        progress();
     }
     itsUsed = cursor;
 }
@@ -1104,6 +1117,39 @@ void OctreeBuilder::copyNodes(HostOctree& o) const
     copy(o.children.data(), itsNodeChild.get(), 32 * m);
     copy(o.flags.data(), itsNodeFlags.get(), m);
     OC_HIP_CHECK(hipStreamSynchronize(itsStream));
+}
+
+void OctreeBuilder::progress()
+{
+    if (itsProgress)
+        itsProgress();
+}
+
+bool OctreeBuilder::snapshotDone()
+{
+    const hipError_t e = hipEventQuery(itsCopiedEvent);
+    if (e == hipErrorNotReady)
+        return false;
+    OC_HIP_CHECK(e);
+    return true;
+}
+
+void OctreeBuilder::drainCopies() noexcept
+{
+    (void)hipStreamSynchronize(itsCopyStream);
+    (void)hipGetLastError();
+}
+
+std::unique_ptr<SlotBuilder> makeSlotBuilder(int device)
+{
+    auto b = std::make_unique<OctreeBuilder>(device);
+    b->setDetailedTimings(false);
+    return b;
+}
+
+uint64_t slotFootprint(const CloudInfo& cloud, const ProcessingInfo& proc, bool hostCloud)
+{
+    return OctreeBuilder::estimateFootprint(cloud, proc, hostCloud);
 }
 
 void OctreeBuilder::waitForSnapshotOnStream()

@@ -258,9 +258,9 @@ int ocpi_get_stage_timings(void* session, float* milliseconds, uint32_t n)
 
 namespace {
 
-oc::BuildService* V(void* h)
+oc::ServiceBase* V(void* h)
 {
-    return oc::BuildService::fromHandle(h);
+    return oc::ServiceBase::fromHandle(h);
 }
 
 }  // namespace
@@ -270,7 +270,16 @@ int ocpi_create_service(void** service, int device, uint32_t slots, uint64_t mem
     return execute([&] {
         if (!service)
             throw std::runtime_error("service handle pointer is null");
-        *service = new oc::BuildService(device, slots, memoryBudget);
+        *service = static_cast<oc::ServiceBase*>(new oc::BuildService(device, slots, memoryBudget));
+    });
+}
+
+int ocpi_create_batch_service(void** service, int device, uint32_t workers, uint32_t maxBatch, uint64_t memoryBudget)
+{
+    return execute([&] {
+        if (!service)
+            throw std::runtime_error("service handle pointer is null");
+        *service = static_cast<oc::ServiceBase*>(new oc::BatchService(device, workers, maxBatch, memoryBudget));
     });
 }
 

@@ -1,9 +1,9 @@
 #include "pinned_pool.h"
 
-#include <hip/hip_runtime.h>
-
 #include <stdexcept>
 #include <string>
+
+#include "slot_builder.h"
 
 namespace oc {
 
@@ -18,7 +18,7 @@ static size_t sizeClass(size_t bytes)
 PinnedPool::~PinnedPool()
 {
     for (auto& kv : itsFree)
-        (void)hipHostFree(kv.second);
+        pinnedRelease(kv.second);
 }
 
 std::shared_ptr<uint8_t> PinnedPool::acquire(size_t bytes)
@@ -36,12 +36,7 @@ std::shared_ptr<uint8_t> PinnedPool::acquire(size_t bytes)
         }
     }
     if (!p)
-    {
-        hipError_t e = hipHostMalloc(&p, capacity, hipHostMallocDefault);
-        if (e != hipSuccess)
-            throw std::runtime_error("hipHostMalloc of " + std::to_string(capacity) +
-                                     " bytes failed: " + hipGetErrorString(e));
-    }
+        p = pinnedAllocate(capacity);  // throws on failure
     std::shared_ptr<PinnedPool> self = shared_from_this();
     return std::shared_ptr<uint8_t>(static_cast<uint8_t*>(p),
                                     [self, capacity](uint8_t* q) { self->recycle(q, capacity); });
@@ -58,7 +53,7 @@ void PinnedPool::recycle(void* p, size_t capacity)
             return;
         }
     }
-    (void)hipHostFree(p);
+    pinnedRelease(p);
 }
 
 }  // namespace oc

@@ -40,9 +40,10 @@ Session::Session(int device) : itsDevice(device)
 
 Session::~Session()
 {
-    if (itsRequest.owner &&
-        (itsRequest.state == Request::QUEUED || itsRequest.state == Request::RUNNING))
-        itsRequest.owner->forget(this);
+    ServiceBase* owner = itsRequest.owner.load();
+    const Request state = itsRequest.state.load();
+    if (owner && (state == Request::QUEUED || state == Request::RUNNING))
+        owner->forget(this);
     if (!itsMemoryReport.empty())
     {
         try

@@ -8,11 +8,12 @@
 #include <memory>
 #include <string>
 
+#include "build_request.h"
 #include "builder.h"
 
 namespace oc {
 
-class Session
+class Session final : public BuildRequest
 {
 public:
     explicit Session(int device);
@@ -48,31 +49,14 @@ public:
     const BuildResult& result() const;
     const HostOctree& hostOctree();  // lazily copied after each build
 
-    // BuildService hooks. A submitted session is owned by the service until its wait returns.
-    void validate() const;
-    const CloudInfo& cloudInfo() const { return itsCloud; }
-    const ProcessingInfo& processingInfo() const { return itsProc; }
-    const uint8_t* hostCloud() const { return itsHostCloud; }
-    const uint8_t* deviceCloud() const { return itsDeviceCloud; }
-    void adoptResult(std::unique_ptr<HostBuildResult> result);
-
-    enum class Request
-    {
-        NONE,
-        QUEUED,
-        RUNNING,
-        DONE,
-        FAILED
-    };
-    struct RequestInfo
-    {
-        Request state = Request::NONE;
-        std::string error;
-        uint64_t footprint = 0;
-        class BuildService* owner = nullptr;  // the service the session was submitted to
-        std::chrono::steady_clock::time_point submitted, started, finished;
-    };
-    RequestInfo& request() { return itsRequest; }
+    // Build service hooks (BuildRequest). A submitted session is owned by the service until its
+    // wait returns.
+    void validate() const override;
+    const CloudInfo& cloudInfo() const override { return itsCloud; }
+    const ProcessingInfo& processingInfo() const override { return itsProc; }
+    const uint8_t* hostCloud() const override { return itsHostCloud; }
+    const uint8_t* deviceCloud() const override { return itsDeviceCloud; }
+    void adoptResult(std::unique_ptr<HostBuildResult> result) override;
 
 private:
 
@@ -85,7 +69,6 @@ private:
     std::unique_ptr<HostBuildResult> itsSnapshot;
     const BuildResult* itsResult = nullptr;
     std::unique_ptr<HostOctree> itsHost;
-    RequestInfo itsRequest;
     uint64_t itsBuildSerial     = 0;
     uint64_t itsHostSerial      = ~0ull;
     std::string itsMemoryReport;

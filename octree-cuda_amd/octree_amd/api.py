@@ -49,6 +49,7 @@ SIGNATURES = {
     "ocpi_copy_output": [_vp, _vp, _u64],
     "ocpi_get_stage_timings": [_vp, _vp, _u32],
     "ocpi_create_service": [ctypes.POINTER(_vp), ctypes.c_int, _u32, _u64],
+    "ocpi_create_batch_service": [ctypes.POINTER(_vp), ctypes.c_int, _u32, _u32, _u64],
     "ocpi_destroy_service": [_vp],
     "ocpi_service_submit": [_vp, _vp],
     "ocpi_service_wait": [_vp, _vp, _u32],
@@ -72,7 +73,7 @@ BATCH_ENTRY_POINTS = ["ocpi_create_batch_builder", "ocpi_destroy_batch_builder",
                       "ocpi_batch_set_launch", "ocpi_batch_use_graphs", "ocpi_batch_get_timings",
                       "ocpi_estimate_batch_footprint"]
 
-SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
+SERVICE_ENTRY_POINTS = ["ocpi_create_service", "ocpi_create_batch_service", "ocpi_destroy_service", "ocpi_service_submit", "ocpi_service_wait",
                         "ocpi_service_set_active_slots", "ocpi_service_get_stats", "ocpi_service_reset_stats",
                         "ocpi_get_request_timing", "ocpi_estimate_footprint"]
 
@@ -86,7 +87,8 @@ SUMMARY_FIELDS = ("sparse_nodes", "points", "capacity", "used", "internal_nodes"
 TIMING_FIELDS = ("count_ms", "merge_ms", "distribute_ms", "subsample_ms", "build_ms", "insert_ms", "emit_ms")
 SERVICE_STAT_FIELDS = ("submitted", "completed", "failed", "queued", "running", "max_queue_depth", "p50_ms", "p99_ms",
                        "mean_ms", "reserved_bytes", "budget_bytes", "active_slots", "slots", "mean_wait_ms",
-                       "max_ms", "peak_reserved_bytes", "setup_ms", "generate_ms", "capture_ms")
+                       "max_ms", "peak_reserved_bytes", "setup_ms", "generate_ms", "capture_ms", "batches",
+                       "requests_per_batch")
 
 _api = None
 
@@ -254,12 +256,21 @@ class Session:
 
 
 class Service:
-    """Continuous batching of builds (ocpi_create_service): configured sessions are requests."""
+    """Continuous batching of builds: configured sessions are requests. The default is the slot
+    service (ocpi_create_service: `slots` single-cloud builders); with batch_size the batch
+    service (ocpi_create_batch_service: `slots` workers that each build up to batch_size queued
+    requests together)."""
 
-    def __init__(self, device: int = 0, slots: int = 4, memory_budget: int = 200 << 30):
+    def __init__(self, device: int = 0, slots: int = 4, memory_budget: int = 200 << 30,
+                 batch_size: int | None = None):
         self._lib = library()
         h = _vp()
-        _check(self._lib.ocpi_create_service(ctypes.byref(h), device, slots, memory_budget), "ocpi_create_service")
+        if batch_size is None:
+            _check(self._lib.ocpi_create_service(ctypes.byref(h), device, slots, memory_budget),
+                   "ocpi_create_service")
+        else:
+            _check(self._lib.ocpi_create_batch_service(ctypes.byref(h), device, slots, batch_size, memory_budget),
+                   "ocpi_create_batch_service")
         self.handle = h
 
     def close(self):

@@ -60,10 +60,15 @@ def _same(a, b, weighted):
     assert d <= (1 if weighted else 0), f"colours differ by {d}"
 
 
-@pytest.mark.parametrize("slots,copies", [(3, 1), (16, 8)])
-def test_service_results_equal_builds_alone(slots, copies):
+def _service(kind, slots, budget):
+    """kind "slots": the slot service; "batch": the batch service with `slots` workers."""
+    return api.Service(0, slots=slots, memory_budget=budget, batch_size=None if kind == "slots" else 16)
+
+
+@pytest.mark.parametrize("kind,slots,copies", [("slots", 3, 1), ("slots", 16, 8), ("batch", 2, 1), ("batch", 3, 8)])
+def test_service_results_equal_builds_alone(kind, slots, copies):
     """copies x the 6 requests through `slots` slots (16 slots and 48 requests: every slot runs its
-    two-deep pipeline of build and result copies several times)."""
+    two-deep pipeline of build and result copies several times) or batch workers."""
     reqs = _requests()
     alone = []
     differing = compared = 0
@@ -84,8 +89,8 @@ def test_service_results_equal_builds_alone(slots, copies):
         compared += c
     assert differing <= WEIGHTED_DIFF_FRACTION * max(compared, 1), (differing, compared)
     mix = [i for _ in range(copies) for i in range(len(reqs))]
-    print("case service mix of", len(mix), "requests,", slots, "slots", flush=True)
-    with api.Service(0, slots=slots, memory_budget=64 << 30) as svc:
+    print("case", kind, "service mix of", len(mix), "requests,", slots, "workers", flush=True)
+    with _service(kind, slots, 64 << 30) as svc:
         sessions = [_session(reqs[i]) for i in mix]
         for s in sessions:
             svc.submit(s)
@@ -101,14 +106,15 @@ def test_service_results_equal_builds_alone(slots, copies):
             s.close()
 
 
-def test_service_keeps_the_memory_budget():
+@pytest.mark.parametrize("kind", ["slots", "batch"])
+def test_service_keeps_the_memory_budget(kind):
     reqs = _requests()[:4]
     sessions = [_session(r) for r in reqs]
     # room for about one request at a time
-    need = max(s.footprint() for s in sessions)
+    need = max(s.footprint() if kind == "slots" else api.batch_footprint([s]) for s in sessions)
     budget = int(need * 1.3)
-    print("case service budget", budget, flush=True)
-    with api.Service(0, slots=3, memory_budget=budget) as svc:
+    print("case", kind, "service budget", budget, flush=True)
+    with _service(kind, 3, budget) as svc:
         for s in sessions:
             svc.submit(s)
         for s in sessions:
@@ -120,22 +126,24 @@ def test_service_keeps_the_memory_budget():
         s.close()
 
 
-def test_service_rejects_requests_above_the_budget():
+@pytest.mark.parametrize("kind", ["slots", "batch"])
+def test_service_rejects_requests_above_the_budget(kind):
     req = _requests()[0]
     s = _session(req)
-    with api.Service(0, slots=2, memory_budget=1 << 20) as svc:
+    with _service(kind, 2, 1 << 20) as svc:
         with pytest.raises(api.OctreeError, match="budget"):
             svc.submit(s)
     s.close()
 
 
-def test_service_failure_stays_with_its_request():
+@pytest.mark.parametrize("kind", ["slots", "batch"])
+def test_service_failure_stays_with_its_request(kind):
     reqs = _requests()[:3]
     cloud, spec, cubic, build = reqs[1]
     import dataclasses
     bad = (cloud, spec, cubic, dataclasses.replace(build, output_factor=1.0))  # N + I > E
     sessions = [_session(reqs[0]), _session(bad), _session(reqs[2])]
-    with api.Service(0, slots=2, memory_budget=64 << 30) as svc:
+    with _service(kind, 2, 64 << 30) as svc:
         for s in sessions:
             svc.submit(s)
         svc.wait(sessions[0])

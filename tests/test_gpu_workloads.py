@@ -115,13 +115,15 @@ def serving_set(per_mode=2, rate=100.0, seed=B.SERVE_SEED):
 
 
 @pytest.mark.timeout(900)
-def test_serving_workload_against_oracle():
+@pytest.mark.parametrize("kind", ["slots", "batch"])
+def test_serving_workload_against_oracle(kind):
     t0 = time.time()
     reqs = serving_set()
-    print(f"case serving workload: {len(reqs)} requests, {sum(r.points for r in reqs):,} points, "
-          f"arrivals over {reqs[-1].arrival:.2f} s, 4 slots", flush=True)
+    print(f"case serving workload ({kind} service): {len(reqs)} requests, {sum(r.points for r in reqs):,} points, "
+          f"arrivals over {reqs[-1].arrival:.2f} s", flush=True)
     sessions = [r.session() for r in reqs]
-    with api.Service(0, slots=4, memory_budget=200 << 30) as svc:
+    with api.Service(0, slots=4 if kind == "slots" else 2, memory_budget=200 << 30,
+                     batch_size=None if kind == "slots" else 32) as svc:
         start = time.perf_counter()
         for r, s in zip(reqs, sessions):
             delay = start + r.arrival - time.perf_counter()
