@@ -513,7 +513,7 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     OC_LAUNCH_CHECK();
     kBucketScanB<T><<<std::min<uint32_t>(B0, 1024), SCAN_THREADS, 0, s>>>(ctx, B0, L.buckets);
     OC_LAUNCH_CHECK();
-    kBucketScatterB<T><<<LG.sortGrid, SORT_THREADS, 0, s>>>(ctx, B0, slot(P_SCATTER), fineBits);
+    kBucketScatterB<T><<<LG.sortGrid, SCATTER_THREADS, 0, s>>>(ctx, B0, slot(P_SCATTER), fineBits);
     OC_LAUNCH_CHECK();
     kFineSortB<T><<<LG.sortGrid, SORT_THREADS, 0, s>>>(ctx, B0, fp, L.buckets);
     OC_LAUNCH_CHECK();

@@ -7,8 +7,9 @@
 //               bucket level (K1)
 //   merge       rest of the count pyramid, canonical sparse ids by scan, node links and flags,
 //               leaf data blocks by scan (K2-K4), cell -> leaf map
-//   distribute  each leaf's range of the sorted cloud -> its data block: point index and
-//               18-byte record, coalesced (K5)
+//   distribute  the records gathered through the permutation into leaf order (and, inside a
+//               leaf, the cell order of its parent's subsampling grid), then each leaf's range of
+//               that sorted cloud -> its data block: point index and 18-byte record, coalesced (K5)
 //   subsample   one batch per octree level (all internal nodes of a level at once) with per-node
 //               occupancy bitmaps and a popcount rank dictionary; winners by atomicMin, colours
 //               by packed integer sums or a gathered distance-weighted filter; then DFS
@@ -30,7 +31,6 @@
 #include "types.h"
 
 namespace oc {
-
 
 class DeviceBuffer
 {
@@ -208,8 +208,6 @@ private:
         EV_COUNT_END,
         EV_MERGE_END,
         EV_DIST_BEGIN,
-        EV_ORDER_BEGIN,
-        EV_ORDER_END,
         EV_DIST_END,
         EV_END,
         EV_SLOTS

@@ -8,7 +8,7 @@
 //   sorted         the same records in level-0 Morton order: every node is one contiguous range;
 //                  the leaves' points are read from here by the subsampling
 //   count          D x u32 dense count pyramid; after the merge its level-0 part maps cell -> leaf
-//   denseToSparse  D x i32
+//   denseToSparse  D x i32, set for occupied cells (non-zero dense count) only
 //   nodes (SoA)    key, dense count, pointCount, dataIdx (u64), parent, children[8], flags, ...
 //   output region  E records: lut (u32 point index), out (18-byte record). [0, N) holds the leaves,
 //                  [N, N + I) the subsampled points in DFS post-order
@@ -452,10 +452,9 @@ void OctreeBuilder::generateTyped()
         return v;
     };
     itsTimings.assign(7, 0.f);
-    const float order = ms(EV_ORDER_BEGIN, EV_ORDER_END);  // moving the records: count stage
-    itsTimings[0]     = ms(EV_COUNT_BEGIN, EV_COUNT_END) + order;
-    itsTimings[1]     = ms(EV_COUNT_END, EV_MERGE_END);
-    itsTimings[2]     = ms(EV_DIST_BEGIN, EV_DIST_END) - order;
+    itsTimings[0] = ms(EV_COUNT_BEGIN, EV_COUNT_END);
+    itsTimings[1] = ms(EV_COUNT_END, EV_MERGE_END);
+    itsTimings[2] = ms(EV_DIST_BEGIN, EV_DIST_END);
     itsTimings[3] = ms(EV_DIST_END, EV_END);
     itsTimings[4] = ms(EV_START, EV_END);
     for (uint32_t i = 0; i < itsBatchesRun; ++i)
@@ -493,7 +492,7 @@ void OctreeBuilder::countPoints()
                             itsScanScratch.as<uint32_t>(), bucketStart + nb, itsStream);
     OC_LAUNCH_CHECK();
     const unsigned scatterBlocks = static_cast<unsigned>((static_cast<uint64_t>(n) + STAGE_CHUNK - 1) / STAGE_CHUNK);
-    kBucketScatter<T><<<scatterBlocks, SORT_THREADS, 0, itsStream>>>(itsCloudDevice, n, itsCloud.dataStride, g,
+    kBucketScatter<T><<<scatterBlocks, SCATTER_THREADS, 0, itsStream>>>(itsCloudDevice, n, itsCloud.dataStride, g,
                                                                      3 * itsCoarse, nb, bucketCursor,
                                                                      itsBucketed.as<SortedPoint<T>>(),
                                                                      itsFineCode.as<uint16_t>());
@@ -519,7 +518,7 @@ void OctreeBuilder::countPoints()
         scanExclusive<uint32_t>(nb, SplitChunksIn{bucketStart}, SplitChunksOut{itsSplitChunks.as<uint64_t>()},
                                 itsScanScratch.as<uint32_t>(), chunkCount, itsStream);
         OC_LAUNCH_CHECK();
-        const unsigned grid = static_cast<unsigned>(bound);
+        const unsigned grid = static_cast<unsigned>((bound + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT);
         kFineChunkHist<<<grid, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart,
                                                              itsSplitChunks.as<uint64_t>(), chunkCount, fp.lc,
                                                              itsSplitHist.as<uint32_t>());
@@ -532,7 +531,7 @@ void OctreeBuilder::countPoints()
                                                              itsSplitHist.as<uint32_t>(), itsPerm.as<uint32_t>());
         OC_LAUNCH_CHECK();
     }
-    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));  // the records move once the leaves are known
+    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));  // the records move with the distribution
 }
 
 void OctreeBuilder::mergeLevels()
@@ -603,7 +602,7 @@ void OctreeBuilder::mergeLevels()
     lp.depth     = itsDepth;
     lp.threshold = itsProc.mergingThreshold;
     kNodeLinks<<<gridFor(m), BLOCK, 0, itsStream>>>(
-            itsM, lp, itsNodeKey.as<uint32_t>(), itsNodeCount.as<uint32_t>(), itsDenseToSparse.as<int32_t>(),
+            itsM, lp, itsNodeKey.as<uint32_t>(), itsNodeCount.as<uint32_t>(), count, itsDenseToSparse.as<int32_t>(),
             itsNodePoints.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsNodeParent.as<int32_t>(),
             itsNodeChild.as<int32_t>(), itsNodeFlags.as<uint8_t>(), itsNodeAux.as<uint32_t>(),
             itsNodeSub.as<uint32_t>());
@@ -645,22 +644,19 @@ void OctreeBuilder::distributePoints()
                                                          itsNodeParent.as<int32_t>(), itsNodeChild.as<int32_t>(),
                                                          itsNodeSorted.as<uint32_t>());
     OC_LAUNCH_CHECK();
-    // the sorted cloud (part of the count stage's work): leaf and cell order when there are
-    // internal nodes, plain Morton order otherwise
-    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_ORDER_BEGIN], itsStream));
+    // the records move to the sorted cloud: leaf and cell order when there are internal nodes,
+    // plain Morton order otherwise; then the leaf data blocks are copied from it
+    const uint32_t n = itsCloud.pointAmount;
     if (!orderLeafPoints<T>())
     {
-        const uint32_t n = itsCloud.pointAmount;
         kGatherSorted<T><<<gatherGrid(n), BLOCK, 0, itsStream>>>(itsBucketed.as<SortedPoint<T>>(),
                                                                      itsPerm.as<uint32_t>(), n,
                                                                      itsSorted.as<SortedPoint<T>>());
         OC_LAUNCH_CHECK();
     }
-    OC_HIP_CHECK(hipEventRecord(itsEvents[EV_ORDER_END], itsStream));
-    kDistributeSorted<T><<<gridFor(itsCloud.pointAmount, 16384), BLOCK, 0, itsStream>>>(
-            itsSorted.as<SortedPoint<T>>(), itsCloud.pointAmount, g, itsCount.as<uint32_t>(),
-            itsNodeSorted.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(),
-            sx, sy, sz);
+    kDistributeSorted<T><<<gridFor(n, 16384), BLOCK, 0, itsStream>>>(
+            itsSorted.as<SortedPoint<T>>(), n, g, itsCount.as<uint32_t>(), itsNodeSorted.as<uint32_t>(),
+            itsNodeData.as<uint64_t>(), itsLut.as<uint32_t>(), itsOut.as<OutRecord>(), sx, sy, sz);
     OC_LAUNCH_CHECK();
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_DIST_END], itsStream));
 }

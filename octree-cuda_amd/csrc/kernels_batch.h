@@ -533,7 +533,7 @@ static __global__ __launch_bounds__(SCAN_THREADS) void kBucketScanB(const BatchC
 
 // items: (cloud, chunk of STAGE_CHUNK points)
 template <typename T>
-static __global__ __launch_bounds__(SORT_THREADS) void kBucketScatterB(const BatchCloud<T>* __restrict__ ctx, uint32_t B,
+static __global__ __launch_bounds__(SCATTER_THREADS) void kBucketScatterB(const BatchCloud<T>* __restrict__ ctx, uint32_t B,
                                                                  const uint32_t* __restrict__ prefix,
                                                                  uint32_t fineBits)
 {
@@ -682,7 +682,7 @@ static __global__ __launch_bounds__(BLOCK) void kNodeLinksB(const BatchCloud<T>*
         const uint32_t c        = findCloud(wp, B, static_cast<uint32_t>(i));
         const BatchCloud<T>& cc = ctx[c];
         const uint32_t id       = static_cast<uint32_t>(i) - wp[c];
-        nodeLinksOf(id, levelParams(L, cc.threshold), cc.nodeKey, cc.nodeCount, cc.denseToSparse, cc.nodePoints,
+        nodeLinksOf(id, levelParams(L, cc.threshold), cc.nodeKey, cc.nodeCount, cc.count, cc.denseToSparse, cc.nodePoints,
                     cc.nodeData, cc.nodeParent, cc.nodeChild, cc.nodeFlags, cc.nodeAux, cc.nodeSub);
         cc.keyOut[id]   = cc.nodeKey[id];
         cc.countOut[id] = cc.nodeCount[id];
@@ -749,18 +749,8 @@ static __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrderB(const Bat
     {
         const uint32_t c        = findCloud(prefix, B, item);
         const BatchCloud<T>& cc = ctx[c];
-        LeafOrderParams p;
-        p.cubicMin[0] = cc.grid.minX;
-        p.cubicMin[1] = cc.grid.minY;
-        p.cubicMin[2] = cc.grid.minZ;
-        for (uint32_t l = 0; l < MAX_LEVELS; ++l)
-        {
-            p.sideHalf[l] = cc.sideHalf[l];
-            p.side[l]     = cc.side[l];
-        }
-        p.lp       = levelParams(L, cc.threshold);
-        p.g        = cc.g;
-        p.keyShift = cc.keyShift;
+        // the cubic minimum is Gridding's first three members
+        const LeafOrderView p{&cc.grid.minX, cc.sideHalf, cc.side, L.offset, L.grid, L.depth, cc.g, cc.keyShift};
         leafCellOrderChunk<T>(cc.chunkList[item - prefix[c]], p, cc.nodeKey, cc.nodeParent, cc.nodeCount,
                               cc.nodeSorted, cc.bucketed, cc.perm, cc.sorted, lds);
     }

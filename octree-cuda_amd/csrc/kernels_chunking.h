@@ -79,6 +79,8 @@ struct OccupiedIn
 
 // Sparse id = exclusive rank of the occupied dense cell (A2); the node's key and dense count are
 // written in the same pass.
+// denseToSparse is written for occupied cells only (a cell is occupied where its dense count is
+// not zero): an array of D entries is never filled with -1.
 struct OccupiedOut
 {
     int32_t* denseToSparse;
@@ -87,9 +89,9 @@ struct OccupiedOut
     const uint32_t* count;
     __device__ void operator()(uint64_t d, uint32_t prefix, uint32_t v) const
     {
-        denseToSparse[d] = v ? static_cast<int32_t>(prefix) : -1;
         if (v)
         {
+            denseToSparse[d]  = static_cast<int32_t>(prefix);
             nodeKey[prefix]   = static_cast<uint32_t>(d);
             nodeCount[prefix] = count[d];
         }
@@ -153,6 +155,7 @@ __device__ __forceinline__ uint32_t levelOfKey(uint32_t key, const LevelParams& 
 // and the size of the leaf data block an internal node hands out to its leaf children.
 __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, const uint32_t* __restrict__ nodeKey,
                                             const uint32_t* __restrict__ nodeCount,
+                                            const uint32_t* __restrict__ denseCount,
                                             const int32_t* __restrict__ denseToSparse,
                                             uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
                                             int32_t* __restrict__ parent, int32_t* __restrict__ children,
@@ -171,7 +174,7 @@ __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, 
         int32_t p = -1;
         if (l < lp.depth)
         {
-            uint32_t pg = lp.grid[l + 1];
+            uint32_t pg = lp.grid[l + 1];  // the parent of an occupied cell is occupied
             p           = denseToSparse[lp.offset[l + 1] + (x >> 1) + (y >> 1) * pg + (z >> 1) * pg * pg];
         }
         bool finished = (p < 0) ? true : (nodeCount[p] >= lp.threshold);
@@ -183,7 +186,8 @@ __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, 
             for (int k = 0; k < 8; ++k)
             {
                 uint32_t cx = 2 * x + ((k >> 2) & 1), cy = 2 * y + ((k >> 1) & 1), cz = 2 * z + (k & 1);
-                int32_t c   = denseToSparse[lp.offset[l - 1] + cx + cy * cg + cz * cg * cg];
+                const uint32_t cd = lp.offset[l - 1] + cx + cy * cg + cz * cg * cg;
+                int32_t c         = denseCount[cd] ? denseToSparse[cd] : -1;  // set for occupied cells only
                 children[8 * static_cast<uint64_t>(id) + k] = c;
                 if (internal && c >= 0)
                 {
@@ -210,6 +214,7 @@ __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, 
 
 static __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelParams lp, const uint32_t* __restrict__ nodeKey,
                                                      const uint32_t* __restrict__ nodeCount,
+                                                     const uint32_t* __restrict__ denseCount,
                                                      const int32_t* __restrict__ denseToSparse,
                                                      uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
                                                      int32_t* __restrict__ parent,
@@ -218,8 +223,8 @@ static __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelPara
                                                      uint32_t* __restrict__ subtree)
 {
     for (uint64_t i = globalThread(); i < m; i += globalStride())
-        nodeLinksOf(static_cast<uint32_t>(i), lp, nodeKey, nodeCount, denseToSparse, pointCount, dataIdx, parent,
-                    children, flags, blockSize, subtree);
+        nodeLinksOf(static_cast<uint32_t>(i), lp, nodeKey, nodeCount, denseCount, denseToSparse, pointCount, dataIdx,
+                    parent, children, flags, blockSize, subtree);
 }
 
 struct ArrayIn

@@ -37,9 +37,24 @@ constexpr uint32_t MAX_FINE_CELLS   = 1u << (3 * MAX_COARSE_LEVEL);    // 32768
 constexpr uint32_t MAX_BUCKETS      = 4096;                            // (512 >> 5)^3
 constexpr int SORT_THREADS          = 1024;
 constexpr int SORT_WAVES            = SORT_THREADS / 64;
+constexpr int SCATTER_THREADS       = 1024;
 constexpr int STAGE_ITEMS           = 4;
-constexpr uint32_t STAGE_CHUNK      = SORT_THREADS * STAGE_ITEMS;      // points per scatter workgroup
+constexpr uint32_t STAGE_CHUNK      = SCATTER_THREADS * STAGE_ITEMS;   // points per scatter workgroup
 constexpr int HIST_THREADS          = 512;
+constexpr uint32_t XCD_COUNT        = 8;  // workgroup b runs on XCD b mod 8, each XCD with its own L2
+
+// The item of a work list of `count` entries that this workgroup takes when XCD x is to take the
+// x-th eighth of the list, front to back (~0u: none). Neighbouring items then share one L2 instead
+// of being spread over eight. The grid must hold at least count rounded up to a multiple of 8.
+__device__ __forceinline__ uint32_t xcdItem(uint32_t count)
+{
+    const uint32_t per = (count + XCD_COUNT - 1) / XCD_COUNT;
+    const uint32_t at  = blockIdx.x / XCD_COUNT;
+    if (at >= per)
+        return ~0u;
+    const uint32_t item = (blockIdx.x % XCD_COUNT) * per + at;
+    return item < count ? item : ~0u;
+}
 
 // A point of the sorted cloud: the original coordinates, its index in the input cloud and its
 // colour (r | g << 8 | b << 16).
@@ -188,8 +203,8 @@ struct ScatterLds
 {
     SortedPoint<T> stage[STAGE_CHUNK];
     uint32_t delta[MAX_BUCKETS];           // counts, then global start - local start
-    uint16_t localStart[MAX_BUCKETS + 1];  // first staged slot of every bucket
-    uint32_t scratch[SORT_WAVES];
+    uint16_t localStart[MAX_BUCKETS];      // first staged slot of every bucket
+    uint32_t scratch[SCATTER_THREADS / 64];
 };
 
 // One chunk of STAGE_CHUNK points (all threads of the workgroup; ends with the LDS free again).
@@ -203,7 +218,7 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
     uint32_t* delta       = lds.delta;
     uint16_t* localStart  = lds.localStart;
     uint32_t* scratch     = lds.scratch;
-    for (uint32_t b = threadIdx.x; b < MAX_BUCKETS; b += SORT_THREADS)
+    for (uint32_t b = threadIdx.x; b < MAX_BUCKETS; b += SCATTER_THREADS)
         delta[b] = 0;
     __syncthreads();
 
@@ -213,7 +228,7 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
 #pragma unroll
     for (int j = 0; j < STAGE_ITEMS; ++j)
     {
-        const uint64_t i = base + static_cast<uint64_t>(j) * SORT_THREADS + threadIdx.x;
+        const uint64_t i = base + static_cast<uint64_t>(j) * SCATTER_THREADS + threadIdx.x;
         code[j]          = ~0u;
         rank[j]          = 0;
         if (i < n)
@@ -232,9 +247,9 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
     }
     __syncthreads();
 
-    // exclusive scan of the bucket counts (MAX_BUCKETS / SORT_THREADS consecutive per thread),
+    // exclusive scan of the bucket counts (MAX_BUCKETS / SCATTER_THREADS consecutive per thread),
     // one global reservation per occupied bucket
-    constexpr int PER = MAX_BUCKETS / SORT_THREADS;
+    constexpr int PER = MAX_BUCKETS / SCATTER_THREADS;
     uint32_t cnt[PER], sum = 0;
 #pragma unroll
     for (int k = 0; k < PER; ++k)
@@ -243,7 +258,7 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
         sum += cnt[k];
     }
     uint32_t total;
-    uint32_t off = sortBlockExclusiveScan(sum, total, scratch);
+    uint32_t off = sortBlockExclusiveScanN<SCATTER_THREADS>(sum, total, scratch);
 #pragma unroll
     for (int k = 0; k < PER; ++k)
     {
@@ -253,8 +268,6 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
             delta[b] = atomicAdd(cursor + b, cnt[k]) - off;
         off += cnt[k];
     }
-    if (threadIdx.x == 0)
-        localStart[MAX_BUCKETS] = static_cast<uint16_t>(total);
     __syncthreads();
     // records into LDS in bucket order; the fine code straight to its final place (this lane
     // already has the point's Morton code)
@@ -272,27 +285,22 @@ __device__ __forceinline__ void bucketScatterChunk(uint32_t chunk, const uint8_t
         }
     __syncthreads();
 
-    for (uint32_t j = threadIdx.x; j < total; j += SORT_THREADS)
+    // a staged record's bucket is recomputed from its coordinates (a few FP64 operations) rather
+    // than searched for among the buckets' first slots (a dozen dependent LDS reads)
+    for (uint32_t j = threadIdx.x; j < total; j += SCATTER_THREADS)
     {
-        uint32_t lo = 0, hi = MAX_BUCKETS;  // the last bucket whose first slot is <= j
-        while (hi - lo > 1)
-        {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (localStart[mid] <= j)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const uint32_t d = j + delta[lo];
-        if (d >= n)
-            continue;  // cannot happen: the histogram and this pass compute the same codes
-        dst[d] = stage[j];
+        const SortedPoint<T> r = stage[j];
+        const uint32_t m       = pointMorton(static_cast<double>(r.x), static_cast<double>(r.y),
+                                             static_cast<double>(r.z), g);  // in the grid: it was staged
+        const uint32_t d       = j + delta[m >> fineBits];
+        if (d < n)  // always: the histogram and this pass compute the same codes
+            dst[d] = r;
     }
     __syncthreads();
 }
 
 template <typename T>
-static __global__ __launch_bounds__(SORT_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
+static __global__ __launch_bounds__(SCATTER_THREADS) void kBucketScatter(const uint8_t* __restrict__ cloud, uint32_t n,
                                                                 uint32_t stride, Gridding g, uint32_t fineBits,
                                                                 uint32_t buckets, uint32_t* __restrict__ cursor,
                                                                 SortedPoint<T>* __restrict__ dst,
@@ -339,22 +347,53 @@ __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& f
     }
 }
 
-// In place: the exclusive prefix of h[0 .. cells), the total in h[cells]. All threads.
+// In place: the exclusive prefix of h[0 .. cells), the total in h[cells]. All threads; cells is a
+// power of two (8^lc). Each thread scans a run of R = cells / SORT_THREADS consecutive entries
+// around one workgroup scan of the runs' sums (a workgroup scan per SORT_THREADS entries would
+// take 32 of them, 64 barriers, per 32^3-cell bucket). A thread walks its run starting at entry
+// t mod R, so the lanes of a wave read different banks (runs of 32 words start in one bank).
 __device__ __forceinline__ void bucketScan(uint32_t* h, uint32_t cells, uint32_t* scratch)
 {
-    uint32_t carry = 0;
-    for (uint32_t r = 0; r < cells; r += SORT_THREADS)
+    if (cells <= SORT_THREADS)
     {
-        const uint32_t c = r + threadIdx.x;
+        const uint32_t c = threadIdx.x;
         const uint32_t v = c < cells ? h[c] : 0u;
         uint32_t total;
         const uint32_t ex = sortBlockExclusiveScan(v, total, scratch);
         if (c < cells)
-            h[c] = carry + ex;
-        carry += total;
+            h[c] = ex;
+        if (threadIdx.x == 0)
+            h[cells] = total;
+        __syncthreads();
+        return;
+    }
+    const uint32_t R    = cells / SORT_THREADS;
+    uint32_t* run       = h + threadIdx.x * R;
+    const uint32_t from = threadIdx.x & (R - 1);
+    // the run's sum, and the sum of its entries before `from` (read last)
+    uint32_t sum = 0, before = 0;
+    for (uint32_t k = 0; k < R; ++k)
+    {
+        const uint32_t i = (from + k) & (R - 1);
+        const uint32_t v = run[i];
+        sum += v;
+        before += i < from ? v : 0u;
+    }
+    uint32_t total;
+    const uint32_t off = sortBlockExclusiveScan(sum, total, scratch);
+    // the same walk: entries from `from` on start at off + before, the wrapped ones at off
+    uint32_t acc = off + before;
+    for (uint32_t k = 0; k < R; ++k)
+    {
+        const uint32_t i = (from + k) & (R - 1);
+        if (i == 0)
+            acc = off;
+        const uint32_t v = run[i];
+        run[i]           = acc;
+        acc += v;
     }
     if (threadIdx.x == 0)
-        h[cells] = carry;
+        h[cells] = total;
     __syncthreads();
 }
 
@@ -507,9 +546,12 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint
 }
 
 // Split path, 2: per split bucket (the workgroup of its part-0 chunk), the bucket's histogram as
-// the sum of its chunks', its scan and dense counts, and each chunk's starting slot per fine cell
-// (the cell's prefix plus the cell's points in the bucket's earlier chunks), in place of the
-// chunk histograms.
+// the sum of its chunks', its scan and dense counts. The permutation is then written by
+// partitions of the bucket's fine cells, one per chunk: partition k starts at the first cell
+// whose points start at or after k * FINE_SPLIT in the bucket, so each writes one contiguous
+// range of the permutation (chunks of points would interleave their runs in every cell). The
+// first chunk's slot receives the cells' starts; word 0 of chunk k's slot (k >= 1) the first cell
+// of partition k.
 // (all threads; chunk is the first chunk of its bucket)
 __device__ __forceinline__ void fineSplitScan(uint32_t chunk, const uint32_t* __restrict__ bucketStart,
                                               const uint64_t* __restrict__ chunks, const FineSortParams& fp,
@@ -528,18 +570,24 @@ __device__ __forceinline__ void fineSplitScan(uint32_t chunk, const uint32_t* __
         h[c] = v;
     }
     __syncthreads();
-    bucketScan(h, cells, scratch);
+    bucketScan(h, cells, scratch);  // every chunk histogram has been read
     bucketCounts(b, fp, h, count);
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
+        first[c] = h[c];
+    for (uint32_t k = 1 + threadIdx.x; k < parts; k += SORT_THREADS)
     {
-        uint32_t slot = h[c];
-        for (uint32_t k = 0; k < parts; ++k)
+        // the first cell c with h[c] >= k * FINE_SPLIT (k * FINE_SPLIT < h[cells], so c <= cells)
+        const uint32_t target = k * FINE_SPLIT;
+        uint32_t lo = 0, hi = cells;
+        while (lo < hi)
         {
-            uint32_t* e   = first + static_cast<uint64_t>(k) * MAX_FINE_CELLS + c;
-            const uint32_t v = *e;
-            *e            = slot;
-            slot += v;
+            const uint32_t mid = (lo + hi) >> 1;
+            if (h[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
         }
+        first[static_cast<uint64_t>(k) * MAX_FINE_CELLS] = lo;
     }
 }
 
@@ -556,7 +604,8 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint
     fineSplitScan(blockIdx.x, bucketStart, chunks, fp, count, chunkHist, h, scratch);
 }
 
-// Split path, 3: one chunk's part of the permutation from its starting slots.
+// Split path, 3: partition `part` of the bucket's fine cells: every point of the bucket is read,
+// those in the partition's cells [cLo, cHi) take their slots from the cells' running starts.
 __device__ __forceinline__ void fineChunkPerm(uint32_t chunk, const uint16_t* __restrict__ fineCode,
                                               const uint32_t* __restrict__ bucketStart,
                                               const uint64_t* __restrict__ chunks, uint32_t lc,
@@ -566,14 +615,34 @@ __device__ __forceinline__ void fineChunkPerm(uint32_t chunk, const uint16_t* __
     const uint32_t cells = 1u << (3 * lc);
     const uint64_t e     = chunks[chunk];
     const uint32_t b = static_cast<uint32_t>(e >> 32), part = static_cast<uint32_t>(e);
-    const uint32_t base = bucketStart[b];
-    const uint32_t lo   = base + part * FINE_SPLIT;
-    const uint32_t hi   = min(bucketStart[b + 1], lo + FINE_SPLIT);
-    const uint32_t* src = chunkHist + static_cast<uint64_t>(chunk) * MAX_FINE_CELLS;
-    for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
-        h[c] = src[c];
+    const uint32_t base  = bucketStart[b];
+    const uint32_t end   = bucketStart[b + 1];
+    const uint32_t parts = (end - base + FINE_SPLIT - 1) / FINE_SPLIT;
+    const uint32_t* start = chunkHist + static_cast<uint64_t>(chunk - part) * MAX_FINE_CELLS;
+    const uint32_t cLo    = part ? chunkHist[static_cast<uint64_t>(chunk) * MAX_FINE_CELLS] : 0u;
+    const uint32_t cHi    = part + 1 < parts ? chunkHist[static_cast<uint64_t>(chunk + 1) * MAX_FINE_CELLS] : cells;
+    const uint32_t span   = cHi - cLo;
+    for (uint32_t c = threadIdx.x; c < span; c += SORT_THREADS)
+        h[c] = start[cLo + c];
     __syncthreads();
-    finePermute(fineCode, lo, hi, h, base, perm);
+    uint32_t i = base + threadIdx.x;
+    for (; i + (FINE_ITEMS - 1) * SORT_THREADS < end; i += FINE_ITEMS * SORT_THREADS)
+    {
+        uint32_t c[FINE_ITEMS];
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            c[k] = static_cast<uint32_t>(fineCode[i + k * SORT_THREADS]) - cLo;  // wraps below the partition
+#pragma unroll
+        for (uint32_t k = 0; k < FINE_ITEMS; ++k)
+            if (c[k] < span)
+                perm[base + atomicAdd(h + c[k], 1u)] = i + k * SORT_THREADS;
+    }
+    for (; i < end; i += SORT_THREADS)
+    {
+        const uint32_t c = static_cast<uint32_t>(fineCode[i]) - cLo;
+        if (c < span)
+            perm[base + atomicAdd(h + c, 1u)] = i;
+    }
 }
 
 static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint16_t* __restrict__ fineCode,
@@ -584,9 +653,11 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint
                                                                 uint32_t* __restrict__ perm)
 {
     __shared__ uint32_t h[MAX_FINE_CELLS];
-    if (blockIdx.x >= *chunkCount)
+    // the partitions of one bucket read the same fine codes: on one XCD, from one L2
+    const uint32_t chunk = xcdItem(*chunkCount);
+    if (chunk == ~0u)
         return;  // the whole workgroup
-    fineChunkPerm(blockIdx.x, fineCode, bucketStart, chunks, lc, chunkHist, perm, h);
+    fineChunkPerm(chunk, fineCode, bucketStart, chunks, lc, chunkHist, perm, h);
 }
 
 // The sorted cloud: gathered records, written in order. The permutation only moves a record
@@ -595,7 +666,6 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkPerm(const uint
 // are fetched into a single L2 and reused by all of its points instead of by up to eight L2s.
 constexpr int GATHER_ITEMS      = 4;
 constexpr uint32_t GATHER_TILE  = BLOCK * GATHER_ITEMS;
-constexpr uint32_t XCD_COUNT    = 8;
 
 // Workgroups of kGatherSorted for n points (a multiple of XCD_COUNT).
 inline unsigned gatherGrid(uint32_t n)

@@ -247,6 +247,32 @@ struct LeafOrderParams
     uint32_t keyShift;  // word keys >> keyShift < LEAF_KEYS
 };
 
+// What one leaf chunk reads of the level tables and node sides, by pointer: into the kernel
+// argument (single cloud) or into the cloud's table in device memory (batches), so that no copy
+// of the per-level arrays is made per chunk.
+struct LeafOrderView
+{
+    const double* cubicMin;  // 3
+    const double* sideHalf;  // per level
+    const double* side;      // per level
+    const uint32_t* offset;  // per level
+    const uint32_t* grid;    // per level
+    uint32_t depth, g, keyShift;
+};
+
+__device__ __forceinline__ LeafOrderView leafOrderView(const LeafOrderParams& p)
+{
+    return LeafOrderView{p.cubicMin, p.sideHalf, p.side, p.lp.offset, p.lp.grid, p.lp.depth, p.g, p.keyShift};
+}
+
+// Keys a leaf chunk can take: the bitmap words of a leaf's octant of the parent grid, shifted.
+__host__ __device__ __forceinline__ uint32_t leafKeyCount(uint32_t g, uint32_t keyShift)
+{
+    const uint32_t h    = g > 1 ? g / 2 : 1;
+    const uint64_t keys = rowsPerNode(h) * ((h + 31) / 32);
+    return static_cast<uint32_t>(((keys - 1) >> keyShift) + 1);
+}
+
 // chunks of a leaf that a subsampling level reads (finished, not internal, with a parent)
 struct LeafIn
 {
@@ -311,7 +337,7 @@ struct LeafOrderLds
 
 // One chunk of one leaf (all threads; the LDS is free again on return).
 template <typename T>
-__device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrderParams& p,
+__device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrderView& p,
                                                    const uint32_t* __restrict__ nodeKey,
                                                    const int32_t* __restrict__ parent,
                                                    const uint32_t* __restrict__ count,
@@ -328,13 +354,15 @@ __device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrd
     const uint32_t src   = sortedStart[leaf] + first;
     const uint32_t cnt   = total - first < LEAF_CHUNK ? total - first : LEAF_CHUNK;
     const uint32_t pkey  = nodeKey[static_cast<uint32_t>(parent[leaf])];
-    const uint32_t pl    = levelOfKey(pkey, p.lp);
+    uint32_t pl = 0;
+    for (uint32_t k = 1; k <= p.depth; ++k)
+        if (pkey >= p.offset[k])
+            pl = k;
     LeafKeyer keyer;
-    keyer.grid = nodeGridding(pkey - p.lp.offset[pl], p.lp.grid[pl], pl, p.lp.depth, p.cubicMin, p.sideHalf,
-                              p.side[pl], p.g);
+    keyer.grid = nodeGridding(pkey - p.offset[pl], p.grid[pl], pl, p.depth, p.cubicMin, p.sideHalf, p.side[pl], p.g);
     keyer.inv  = keyer.grid.gridSize / keyer.grid.bbSize;
     // the leaf's octant of the parent grid: the low bit of its coordinates one level down
-    const uint32_t ll = pl - 1, lgl = p.lp.grid[ll], llin = nodeKey[leaf] - p.lp.offset[ll];
+    const uint32_t ll = pl - 1, lgl = p.grid[ll], llin = nodeKey[leaf] - p.offset[ll];
     keyer.h           = p.g > 1 ? p.g / 2 : 1;  // cells per axis of the leaf's octant (1 for a 1-cell grid)
     keyer.ox          = ((llin % lgl) & 1u) * keyer.h;
     keyer.oy          = (((llin / lgl) % lgl) & 1u) * keyer.h;
@@ -342,7 +370,9 @@ __device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrd
     keyer.wordsPerRow = (keyer.h + 31) / 32;
     keyer.shift       = p.keyShift;
 
-    for (uint32_t i = threadIdx.x; i < LEAF_KEYS + LEAF_KEYS / 32; i += LEAF_THREADS)
+    // only the keys this grid can produce take part (1024 of the 8192 at g = 64)
+    const uint32_t keys = leafKeyCount(p.g, p.keyShift);
+    for (uint32_t i = threadIdx.x; i < leafSlot(keys); i += LEAF_THREADS)
         hist[i] = 0;
     __syncthreads();
     // all loads in flight first, then the keys and their ranks within the key
@@ -374,22 +404,24 @@ __device__ __forceinline__ void leafCellOrderChunk(uint64_t entry, const LeafOrd
     }
     __syncthreads();
 
-    // exclusive scan of the histogram, LEAF_KEYS / LEAF_THREADS consecutive keys per thread
+    // exclusive scan of the histogram, up to LEAF_KEYS / LEAF_THREADS consecutive keys per thread
     constexpr uint32_t PER = LEAF_KEYS / LEAF_THREADS;
     const uint32_t base    = threadIdx.x * PER;
     uint32_t sum           = 0;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k)
-        sum += hist[leafSlot(base + k)];
+        if (base + k < keys)
+            sum += hist[leafSlot(base + k)];
     uint32_t all;
     uint32_t off = sortBlockExclusiveScanN<LEAF_THREADS>(sum, all, scanLds);
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k)
-    {
-        const uint32_t v         = hist[leafSlot(base + k)];
-        hist[leafSlot(base + k)] = off;
-        off += v;
-    }
+        if (base + k < keys)
+        {
+            const uint32_t v         = hist[leafSlot(base + k)];
+            hist[leafSlot(base + k)] = off;
+            off += v;
+        }
     __syncthreads();
 
 #pragma unroll
@@ -410,7 +442,8 @@ static __global__ __launch_bounds__(LEAF_THREADS) void kLeafCellOrder(const uint
                                                                 SortedPoint<T>* __restrict__ out)
 {
     __shared__ LeafOrderLds lds;
-    leafCellOrderChunk<T>(chunkList[blockIdx.x], p, nodeKey, parent, count, sortedStart, bucketed, perm, out, lds);
+    leafCellOrderChunk<T>(chunkList[blockIdx.x], leafOrderView(p), nodeKey, parent, count, sortedStart, bucketed, perm,
+                          out, lds);
 }
 
 // packed (tiles << 32 | inputs) per segment, so one scan yields both prefix sums

@@ -155,7 +155,7 @@ void ServiceCore::resetStats()
 // ---- slot service -------------------------------------------------------------------------------
 
 BuildService::BuildService(int device, uint32_t slots, uint64_t memoryBudget)
-    : ServiceCore(memoryBudget, slots), itsDevice(device)
+    : ServiceCore(memoryBudget, slots)
 {
     if (slots == 0 || slots > 64)
         throw std::runtime_error("a build service needs 1 to 64 slots");

@@ -116,7 +116,6 @@ private:
     uint64_t reservedLocked() const override { return reservedExcept(~0u); }
     uint32_t runningLocked() const override;
 
-    int itsDevice;
     std::vector<std::unique_ptr<Slot>> itsSlots;
     std::shared_ptr<PinnedPool> itsPinned;  // results are copied into page-locked buffers
 };

@@ -1,6 +1,7 @@
 // Plain data shared by host and device code.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace oc {
@@ -40,6 +41,8 @@ struct Gridding
     double bbSize;
     uint32_t gridSize;
 };
+static_assert(offsetof(Gridding, minY) == sizeof(double) && offsetof(Gridding, minZ) == 2 * sizeof(double),
+              "the cubic minimum is read as three consecutive doubles");
 
 // Per internal node parameters of one subsampling level.
 struct NodeGrid

@@ -326,22 +326,40 @@ def run_serving(service: api.Service, requests, duration: float, sample_every: f
 
 
 def main(argv=None):
-    """Command line: build the batched workload on the service, e.g. under a kernel trace:
-    `python3 -m octree_amd.batch --slots 8 --repeats 2` (run from octree-cuda_amd/, or with it on
-    PYTHONPATH). The first pass grows the slots' arenas; every pass prints one JSON line."""
+    """Command line: build the batched workload, e.g. under a kernel trace:
+    `python3 -m octree_amd.batch --builder batch --repeats 3` (run from octree-cuda_amd/, or with it
+    on PYTHONPATH). `--builder batch` runs the batched pipeline (one launch per stage for all clouds,
+    clouds resident in device memory, graph replay, the launch shape given by --launch);
+    `--builder slots` the slot service with --slots slots. The first pass grows the arenas; every
+    pass prints one JSON line."""
     import argparse
     import json
 
     ap = argparse.ArgumentParser(prog="python3 -m octree_amd.batch")
+    ap.add_argument("--builder", choices=("batch", "slots"), default="batch")
     ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--launch", default="8192,16384,2048", help="batched launch shape: thread,tile,sort grids")
     ap.add_argument("--repeats", type=int, default=2, help="passes over the batch (the first warms the arenas)")
     ap.add_argument("--device", type=int, default=0)
     args = ap.parse_args(argv)
     reqs = batch_requests()
-    with api.Service(args.device, slots=args.slots, memory_budget=200 << 30) as svc:
-        for rep in range(args.repeats):
-            r = run_batch(svc, reqs, args.slots)
-            print(json.dumps({"pass": rep, **r}), flush=True)
+    if args.builder == "slots":
+        with api.Service(args.device, slots=args.slots, memory_budget=200 << 30) as svc:
+            for rep in range(args.repeats):
+                r = run_batch(svc, reqs, args.slots)
+                print(json.dumps({"pass": rep, **r}), flush=True)
+        return
+    thread, tile, sort = (int(v) for v in args.launch.split(","))
+    clouds, addresses = upload_clouds(reqs)
+    try:
+        with api.BatchBuilder(args.device) as bb:
+            bb.set_launch(thread, tile, sort)
+            bb.use_graphs(True)
+            for rep in range(args.repeats):
+                r = run_batch_builder(bb, reqs, addresses)
+                print(json.dumps({"pass": rep, **r, "stage_ms": bb.timings()}), flush=True)
+    finally:
+        clouds.close()
 
 
 if __name__ == "__main__":
