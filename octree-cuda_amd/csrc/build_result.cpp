@@ -4,6 +4,7 @@
 #include <stdexcept>
 #include <type_traits>
 
+#include "host_copy.h"
 #include "pinned_pool.h"
 
 namespace oc {
@@ -115,7 +116,7 @@ void HostBuildResult::copyLut(uint32_t* dst, uint64_t count) const
         throw std::runtime_error("copy beyond the output capacity");
     const uint64_t head = count < itsUsed ? count : itsUsed;
     if (head)
-        std::memcpy(dst, itsLut, head * sizeof(uint32_t));
+        parallelMemcpy(dst, itsLut, head * sizeof(uint32_t));
     if (count > head)
         std::memset(dst + head, 0, (count - head) * sizeof(uint32_t));
 }
@@ -128,7 +129,7 @@ void HostBuildResult::copyOutputRange(uint8_t* dst, uint64_t first, uint64_t cou
     if (first < itsUsed)
     {
         copied = (first + count <= itsUsed ? count : itsUsed - first);
-        std::memcpy(dst, itsOut + first * sizeof(OutRecord), copied * sizeof(OutRecord));
+        parallelMemcpy(dst, itsOut + first * sizeof(OutRecord), copied * sizeof(OutRecord));
     }
     if (count > copied)
         std::memset(dst + copied * sizeof(OutRecord), 0, (count - copied) * sizeof(OutRecord));

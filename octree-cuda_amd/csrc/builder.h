@@ -103,6 +103,11 @@ public:
     void copyNodes(HostOctree& out) const override;
     void copyLut(uint32_t* dst, uint64_t count) const override;
     void copyOutputRange(uint8_t* dst, uint64_t first, uint64_t count) const override;
+    // device bytes at src to host memory at dst (synchronous; staged for large pageable copies)
+    void download(void* dst, const void* src, uint64_t bytes) const;
+    // host bytes at src to device memory at dst on the builder's stream (staged, and then
+    // synchronous, for large pageable sources)
+    void upload(void* dst, const void* src, uint64_t bytes);
     void copySnapshot(uint8_t* dst) const override;
     // The snapshot copies on the builder's copy stream, after the build: the host goes on (the
     // next setup's upload overlaps them); the next setup / generate makes the build stream wait

@@ -8,6 +8,8 @@
 //                     goes on; snapshotDone / waitSnapshot / drainCopies observe or join it
 //   StubBatchBuilder  builds a batch of requests, one snapshot each
 //   pinned memory     malloc / free
+// and the threaded host copy that moves large results into callers' memory is run by several
+// callers at once.
 // Every result's content is a function of its request, so a result handed to the wrong session,
 // read before its copy finished or written after its block was recycled shows up as a wrong value
 // or a sanitizer report. Requests fail on purpose (in validation, in generate, by budget), slots
@@ -26,6 +28,7 @@
 
 #include "build_request.h"
 #include "build_result.h"
+#include "host_copy.h"
 #include "pinned_pool.h"
 #include "service.h"
 #include "slot_builder.h"
@@ -461,6 +464,27 @@ int main()
         const std::vector<double> st = svc.stats();
         if (st[15] > 5000)
             fail("peak reserved memory above the budget");
+    }
+    {
+        // the threaded host copy of large results (host_copy.cpp): three callers at once, sizes
+        // that are not multiples of its parts, into fresh destinations
+        std::vector<std::thread> callers;
+        for (uint32_t t = 0; t < 3; ++t)
+            callers.emplace_back([t] {
+                for (uint32_t k = 0; k < 4; ++k)
+                {
+                    const size_t bytes = (9u << 20) + 4099u * (t + 1) * (k + 1);
+                    std::vector<uint8_t> src(bytes);
+                    for (size_t i = 0; i < bytes; ++i)
+                        src[i] = static_cast<uint8_t>((i * 131u + t * 7u + k) >> 3);
+                    std::unique_ptr<uint8_t[]> dst(new uint8_t[bytes]);
+                    parallelMemcpy(dst.get(), src.data(), bytes);
+                    if (std::memcmp(dst.get(), src.data(), bytes) != 0)
+                        fail("threaded host copy differs from its source");
+                }
+            });
+        for (auto& c : callers)
+            c.join();
     }
     std::fprintf(stderr, "harness: %d failures\n", gFailures.load());
     return gFailures.load() == 0 ? 0 : 1;

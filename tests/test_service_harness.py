@@ -11,7 +11,7 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(REPO, "octree-cuda_amd", "csrc")
-SOURCES = [os.path.join(CSRC, f) for f in ("service.cpp", "pinned_pool.cpp", "build_result.cpp", "log.cpp")] + \
+SOURCES = [os.path.join(CSRC, f) for f in ("service.cpp", "pinned_pool.cpp", "build_result.cpp", "host_copy.cpp", "log.cpp")] + \
           [os.path.join(REPO, "tests", "service_harness.cpp")]
 
 # clang's ThreadSanitizer: g++ 11's misses the unlock inside a timed condition-variable wait
