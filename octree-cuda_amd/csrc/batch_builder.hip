@@ -23,6 +23,7 @@
 #include <string>
 #include <tuple>
 
+#include "host_staging.h"
 #include "kernels_batch.h"
 
 namespace oc {
@@ -413,6 +414,7 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
     carveAll(itsArena.as<uint8_t>());
     uint64_t* scalarsBase = table[0].sc;
 
+    std::vector<HostSegment> uploads;
     for (uint32_t i = 0; i < B0; ++i)
     {
         const BatchRequest& r   = requests[members[i]];
@@ -445,11 +447,12 @@ void BatchBuilder::buildGroup(const std::vector<uint32_t>& members, const std::v
             cc.cloud = r.deviceCloud;
         else
         {
-            OC_HIP_CHECK(hipMemcpyAsync(cloudCopy[i], r.hostCloud, sz.n * r.cloud.dataStride, hipMemcpyHostToDevice,
-                                        s));
+            uploads.push_back({cloudCopy[i], r.hostCloud, static_cast<uint64_t>(sz.n) * r.cloud.dataStride});
             cc.cloud = cloudCopy[i];
         }
     }
+    // the group's host clouds, staged together
+    uploadSegments(uploads.data(), uploads.size(), s);
 
     // ---- tables ------------------------------------------------------------------------------
     itsTable.ensure(sizeof(BatchCloud<T>) * MAX_BATCH);

@@ -23,17 +23,15 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <mutex>
 #include <stdexcept>
 #include <string>
 
 #include "device_common.h"
-#include "host_copy.h"
+#include "host_staging.h"
 #include "kernels_chunking.h"
 #include "kernels_sort.h"
 #include "kernels_subsample.h"
 #include "scan.h"
-#include "slot_builder.h"
 
 namespace oc {
 
@@ -1197,133 +1195,19 @@ void OctreeBuilder::snapshotAsync(uint8_t* dst)
     itsSnapshotPending = true;
 }
 
-namespace {
-
-// Two page-locked staging buffers shared by every builder of the process (one large upload or
-// download at a time goes through them).
-constexpr uint64_t STAGE_BYTES    = 32ull << 20;
-constexpr uint64_t STAGE_MIN_COPY = 8ull << 20;  // smaller copies go straight to their destination
-
-struct HostStaging
-{
-    std::mutex mutex;
-    uint8_t* buffer[2] = {nullptr, nullptr};
-};
-
-HostStaging& hostStaging()
-{
-    static HostStaging s;  // the buffers live until the process ends
-    return s;
-}
-
-// true when p is page-locked host memory the copy engines can write directly
-bool pageLocked(const void* p)
-{
-    hipPointerAttribute_t attr{};
-    if (hipPointerGetAttributes(&attr, p) != hipSuccess)
-    {
-        (void)hipGetLastError();  // pageable memory: the query fails; clear it
-        return false;
-    }
-    return attr.type == hipMemoryTypeHost;
-}
-
-}  // namespace
-
-// Device to host. Into pageable memory the bytes go through the two staging buffers in chunks:
-// the copy engine fills one while host threads (parallelMemcpy) empty the other into the caller's
-// memory, so the page faults of a fresh destination are taken by several threads while the next
-// chunk is in flight.
+// Device to host, synchronous; large copies into pageable memory are staged (host_staging.h).
 void OctreeBuilder::download(void* dst, const void* src, uint64_t bytes) const
 {
     OC_HIP_CHECK(hipSetDevice(itsDevice));
-    if (bytes < STAGE_MIN_COPY || pageLocked(dst))
-    {
-        OC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, itsStream));
-        OC_HIP_CHECK(hipStreamSynchronize(itsStream));
-        return;
-    }
-    HostStaging& st = hostStaging();
-    std::lock_guard<std::mutex> lock(st.mutex);
-    for (auto& b : st.buffer)
-        if (!b)
-            b = static_cast<uint8_t*>(pinnedAllocate(STAGE_BYTES));
-    hipEvent_t landed[2] = {};
-    for (auto& e : landed)
-        OC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    const auto* from   = static_cast<const uint8_t*>(src);
-    auto* to           = static_cast<uint8_t*>(dst);
-    const uint64_t n   = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
-    hipError_t failure = hipSuccess;
-    auto issue         = [&](uint64_t k) {
-        const uint64_t off = k * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
-        hipError_t e       = hipMemcpyAsync(st.buffer[k & 1], from + off, len, hipMemcpyDeviceToHost, itsStream);
-        if (e == hipSuccess)
-            e = hipEventRecord(landed[k & 1], itsStream);
-        return e;
-    };
-    failure = issue(0);
-    for (uint64_t k = 0; k < n && failure == hipSuccess; ++k)
-    {
-        // chunk k + 1 goes to the buffer chunk k - 1 was emptied from
-        if (k + 1 < n)
-            failure = issue(k + 1);
-        if (failure == hipSuccess)
-            failure = hipEventSynchronize(landed[k & 1]);
-        if (failure == hipSuccess)
-        {
-            const uint64_t off = k * STAGE_BYTES;
-            parallelMemcpy(to + off, st.buffer[k & 1], std::min(STAGE_BYTES, bytes - off));
-        }
-    }
-    if (failure != hipSuccess)
-        (void)hipStreamSynchronize(itsStream);  // no copy may still write a staging buffer
-    for (auto& e : landed)
-        (void)hipEventDestroy(e);
-    OC_HIP_CHECK(failure);
+    downloadStaged(dst, src, bytes, itsStream);
 }
 
-// Host to device. From pageable memory the bytes go through the two staging buffers in chunks:
-// host threads fill one while the copy engine empties the other. Returns once the bytes have
-// left the staging buffers (the copies are ordered on the builder's stream).
+// Host to device on the builder's stream; large copies from pageable memory are staged and have
+// left the host memory when this returns (host_staging.h).
 void OctreeBuilder::upload(void* dst, const void* src, uint64_t bytes)
 {
-    if (bytes < STAGE_MIN_COPY || pageLocked(src))
-    {
-        OC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, itsStream));
-        return;
-    }
-    HostStaging& st = hostStaging();
-    std::lock_guard<std::mutex> lock(st.mutex);
-    for (auto& b : st.buffer)
-        if (!b)
-            b = static_cast<uint8_t*>(pinnedAllocate(STAGE_BYTES));
-    hipEvent_t drained[2] = {};
-    for (auto& e : drained)
-        OC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    const auto* from   = static_cast<const uint8_t*>(src);
-    auto* to           = static_cast<uint8_t*>(dst);
-    const uint64_t n   = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
-    hipError_t failure = hipSuccess;
-    for (uint64_t k = 0; k < n && failure == hipSuccess; ++k)
-    {
-        // the buffer of chunk k - 2 must have been copied out before chunk k fills it
-        if (k >= 2)
-            failure = hipEventSynchronize(drained[k & 1]);
-        if (failure != hipSuccess)
-            break;
-        const uint64_t off = k * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
-        parallelMemcpy(st.buffer[k & 1], from + off, len);
-        failure = hipMemcpyAsync(to + off, st.buffer[k & 1], len, hipMemcpyHostToDevice, itsStream);
-        if (failure == hipSuccess)
-            failure = hipEventRecord(drained[k & 1], itsStream);
-    }
-    // the staging buffers are shared: no copy may read them once the lock is released
-    const hipError_t drain = hipStreamSynchronize(itsStream);
-    for (auto& e : drained)
-        (void)hipEventDestroy(e);
-    OC_HIP_CHECK(failure);
-    OC_HIP_CHECK(drain);
+    const HostSegment seg{dst, src, bytes};
+    uploadSegments(&seg, 1, itsStream);
 }
 
 void OctreeBuilder::copyLut(uint32_t* dst, uint64_t count) const

@@ -11,4 +11,14 @@ namespace oc {
 // copies run on the calling thread alone.
 void parallelMemcpy(void* dst, const void* src, size_t bytes);
 
+struct CopyPiece
+{
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+// every piece's memcpy, the pieces' bytes shared out over the threads as one copy
+void parallelMemcpy(const CopyPiece* pieces, size_t count);
+
 }  // namespace oc
