@@ -1028,34 +1028,64 @@ __device__ __forceinline__ void emitWeightedCell(uint32_t c, uint32_t nodeBase, 
     const float kexp   = static_cast<float>((-10.0 * 1.4426950408889634) / (ng.diag * ng.diag));
     const uint32_t cxm = (static_cast<uint32_t>(cx) + 3u) & 3u;  // (c.x - 1) mod 4: d.x = ((m.x - cxm) & 3) - 1
     float acc[4]       = {0.f, 0.f, 0.f, 0.f};
+    // payloads as 16-byte vectors (they stay in registers under conditional loads)
+    const uint4* pv = reinterpret_cast<const uint4*>(payload);
+    auto weigh      = [&](const uint4& pl, float uyc, float uzc) {
+        const int32_t dx = static_cast<int32_t>(((pl.w >> 24) + 4u - cxm) & 3u) - 1;
+        const float ux   = __builtin_fmaf(-static_cast<float>(dx), tf, -__uint_as_float(pl.x));
+        const float uy   = uyc - __uint_as_float(pl.y);
+        const float uz   = uzc - __uint_as_float(pl.z);
+        float d2         = ux * ux;
+        d2               = __builtin_fmaf(uy, uy, d2);
+        d2               = __builtin_fmaf(uz, uz, d2);
+        const float w    = __builtin_amdgcn_exp2f(d2 * kexp);
+        acc[0]           = __builtin_fmaf(static_cast<float>(pl.w & 0xFF), w, acc[0]);
+        acc[1]           = __builtin_fmaf(static_cast<float>((pl.w >> 8) & 0xFF), w, acc[1]);
+        acc[2]           = __builtin_fmaf(static_cast<float>((pl.w >> 16) & 0xFF), w, acc[2]);
+        acc[3] += w;
+    };
+    // The first two payloads of every row are loaded one row ahead: a row's loop is short (a few
+    // payloads), so without this each row would wait out a full memory latency of its own.
+    uint4 a0{}, a1{};
+    bool v0 = lo[0] < hi[0], v1 = lo[0] + 1 < hi[0];
+    if (v0)
+        a0 = pv[lo[0]];
+    if (v1)
+        a1 = pv[lo[0] + 1];
+#pragma unroll
     for (int r = 0; r < 9; ++r)
     {
+        uint4 b0{}, b1{};
+        bool u0 = false, u1 = false;
+        if (r + 1 < 9)
+        {
+            u0 = lo[r + 1] < hi[r + 1];
+            u1 = lo[r + 1] + 1 < hi[r + 1];
+            if (u0)
+                b0 = pv[lo[r + 1]];
+            if (u1)
+                b1 = pv[lo[r + 1] + 1];
+        }
         const float uyc = -static_cast<float>((r / 3) - 1) * tf;  // -d.y t
         const float uzc = -static_cast<float>((r % 3) - 1) * tf;  // -d.z t
-        auto weigh      = [&](const WeightPayload& pl) {
-            const int32_t dx = static_cast<int32_t>(((pl.rgbx >> 24) + 4u - cxm) & 3u) - 1;
-            const float ux   = __builtin_fmaf(-static_cast<float>(dx), tf, -pl.ox);
-            const float uy   = uyc - pl.oy;
-            const float uz   = uzc - pl.oz;
-            float d2         = ux * ux;
-            d2               = __builtin_fmaf(uy, uy, d2);
-            d2               = __builtin_fmaf(uz, uz, d2);
-            const float w    = __builtin_amdgcn_exp2f(d2 * kexp);
-            acc[0]           = __builtin_fmaf(static_cast<float>(pl.rgbx & 0xFF), w, acc[0]);
-            acc[1]           = __builtin_fmaf(static_cast<float>((pl.rgbx >> 8) & 0xFF), w, acc[1]);
-            acc[2]           = __builtin_fmaf(static_cast<float>((pl.rgbx >> 16) & 0xFF), w, acc[2]);
-            acc[3] += w;
-        };
-        uint32_t q = lo[r];
+        if (v0)
+            weigh(a0, uyc, uzc);
+        if (v1)
+            weigh(a1, uyc, uzc);
+        uint32_t q = lo[r] + 2;
         for (; q + 1 < hi[r]; q += 2)
         {
-            const WeightPayload p0 = payload[q];
-            const WeightPayload p1 = payload[q + 1];
-            weigh(p0);
-            weigh(p1);
+            const uint4 p0 = pv[q];
+            const uint4 p1 = pv[q + 1];
+            weigh(p0, uyc, uzc);
+            weigh(p1, uyc, uzc);
         }
         if (q < hi[r])
-            weigh(payload[q]);
+            weigh(pv[q], uyc, uzc);
+        a0 = b0;
+        a1 = b1;
+        v0 = u0;
+        v1 = u1;
     }
     // the colour (averages of 8-bit channels: at most 255); the record itself is quantised from
     // the staged coordinates by the final placement (kPermute)
