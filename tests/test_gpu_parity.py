@@ -6,6 +6,7 @@ distance-weighted filter (float sums in a different order and a different expf: 
 """
 import itertools
 import json
+import re
 
 import numpy as np
 import pytest
@@ -145,6 +146,52 @@ def test_statistics_report_matches_oracle(tmp_path):
     want_hist = np.bincount((leaf_counts - leaf_counts.min()) // 500)
     np.testing.assert_array_equal(hist[500], want_hist, "histogram with bin width 500")
     assert hist[0].sum() == want["leaves"]
+
+
+def test_memory_report_and_automatic_histogram_width(tmp_path):
+    """memory_report.html and the memory figures of statistics.json are the library's device
+    allocation counters: at the end of the build at least the session's resident buffers (the
+    uploaded cloud, the two dense grids of D cells, LUT and output of E records) are allocated,
+    and at most the peak. The automatic histogram width (bin width 0: Scott's rule, 3.5 stdev /
+    leaves^(1/3) rounded up, histogram.cu:44-48) bins the oracle's leaf counts: the width in the
+    label is that of the oracle's statistics and every bin holds the leaves of its range."""
+    print("case memory report and automatic histogram width 300000 f32 G=256 g=64 T=5000", flush=True)
+    cloud, spec, cubic = synthetic(300_000, 0, box=(100.0, 100.0, 20.0), clusters=8, seed=17)
+    build = _spec(256, 64, 5000, MODES[2])
+    orc = O.build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    assert orc.status == 0
+    want = O.node_statistics(orc.point_count, orc.children, orc.flags, len(orc.key) - 1)
+    s = gpu_build(cloud, spec.points, spec.stride, 0, cubic, [spec.scale] * 3, build)
+    try:
+        summ = s.summary()
+        s.export_memory_report(str(tmp_path / "memory_report.html"))
+        s.export_json_report(str(tmp_path / "statistics.json"))
+        s.export_distribution_histogram(str(tmp_path / "hist.html"), 0)
+    finally:
+        s.close()
+    text = (tmp_path / "memory_report.html").read_text()
+    peak = float(re.search(r"Peak device allocation: ([0-9.eE+-]+) MB", text).group(1))
+    now = float(re.search(r"Allocated at session end: ([0-9.eE+-]+) MB", text).group(1))
+    mem = json.loads((tmp_path / "statistics.json").read_text())["memory"]
+    assert peak == pytest.approx(mem["peak"] / 1e6, rel=1e-4)
+    assert now == pytest.approx(mem["allocated"] / 1e6, rel=1e-4)
+    resident = spec.points * spec.stride + 2 * 4 * summ["dense_nodes"] + summ["capacity"] * (4 + 18)
+    assert mem["allocated"] >= resident, (mem["allocated"], resident)
+    assert mem["allocated"] <= mem["peak"]
+
+    html = (tmp_path / "hist.html").read_text()
+    width = int(re.search(r"binWidth\((\d+)\)", html).group(1))
+    scott = 3.5 * want["stdev"] / want["leaves"] ** (1.0 / 3.0)
+    if abs(scott - round(scott)) > 1e-3:  # away from a rounding boundary the width is exact
+        assert width == int(np.ceil(scott)), (width, scott)
+    else:
+        assert abs(width - scott) <= 1.0
+    data = html[html.index("data:[") + 6:]
+    got = np.array([int(v) for v in data[:data.index("]")].split(",")], np.int64)
+    leaf_counts = orc.point_count[orc.finished & ~orc.internal].astype(np.int64)
+    low = int(leaf_counts.min())
+    expected = np.bincount((leaf_counts - low) // width, minlength=(int(leaf_counts.max()) - low) // width + 1)
+    np.testing.assert_array_equal(got, expected, f"histogram with the automatic width {width}")
 
 
 @pytest.mark.parametrize("G,g,T,n,mode,F", [
