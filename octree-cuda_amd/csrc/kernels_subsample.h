@@ -905,8 +905,15 @@ __device__ __forceinline__ void emitCell(uint32_t c, uint32_t nodeBase, uint64_t
                     v |= static_cast<uint64_t>(bitmap[w0 + 1]) << 32;
                 const uint32_t m     = static_cast<uint32_t>(v >> b0) & ((1u << span) - 1u);
                 const uint32_t first = prefix[w0] + __popc(static_cast<uint32_t>(v) & ((1u << b0) - 1u));
-                for (uint32_t i = 0, n = __popc(m); i < n; ++i)
-                    sum += color[first + i];
+                // at most 3 occupied cells (x - 1 .. x + 1): predicated loads, no loop, so the
+                // rows' loads are all in flight together
+                const uint32_t n = __popc(m);
+                if (n > 0)
+                    sum += color[first];
+                if (n > 1)
+                    sum += color[first + 1];
+                if (n > 2)
+                    sum += color[first + 2];
             }
             decodeColors(sum, rec.r, rec.g, rec.b);
         }
