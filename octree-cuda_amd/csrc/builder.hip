@@ -867,8 +867,8 @@ void OctreeBuilder::subsampleLevels()
                 OC_HIP_CHECK(hipEventRecord(ev[0], itsStream));
             if (blocks > 0)
             {
-                kMark<T><<<blocks, TILE, 0, itsStream>>>(
-                        tileLo, k0, wordsPerNode, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
+                kMark<T><<<xcdRunGrid(blocks, XCD_TILE_RUN), TILE, 0, itsStream>>>(
+                        tileLo, blocks, k0, wordsPerNode, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                         itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                         itsNodeGrid.as<NodeGrid>(), ps, bitmap, itsCellOf.as<uint32_t>(), err);
                 OC_LAUNCH_CHECK();
@@ -955,15 +955,15 @@ void OctreeBuilder::subsampleLevels()
                 if (blocks > 0)
                 {
                     if (itsProc.useRandomSubsampling)
-                        kScatterPayload<T, true><<<blocks, TILE, 0, itsStream>>>(
-                                tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
+                        kScatterPayload<T, true><<<xcdRunGrid(blocks, XCD_TILE_RUN), TILE, 0, itsStream>>>(
+                                tileLo, blocks, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCellKey.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
                                 itsPayload.as<WeightPayload>(), winKey);
                     else
-                        kScatterPayload<T, false><<<blocks, TILE, 0, itsStream>>>(
-                                tileLo, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
+                        kScatterPayload<T, false><<<xcdRunGrid(blocks, XCD_TILE_RUN), TILE, 0, itsStream>>>(
+                                tileLo, blocks, itsTileList.as<uint32_t>(), itsSegTile.as<uint32_t>(),
                                 itsSegCount.as<uint32_t>(), itsSegSrc.as<uint32_t>(), itsSegInput.as<uint32_t>(),
                                 itsNodeGrid.as<NodeGrid>(), itsCellKey.as<uint32_t>(), itsRankOf.as<uint32_t>(),
                                 itsCellStart.as<uint32_t>(), itsCellOf.as<uint32_t>(), ps,
@@ -976,8 +976,7 @@ void OctreeBuilder::subsampleLevels()
                 OC_HIP_CHECK(hipEventRecord(ev[2], itsStream));
             if (weighted)
             {
-                const uint32_t emitBlocks = static_cast<uint32_t>(((cells + BLOCK - 1) / BLOCK + 8 * EMIT_CHUNK - 1) /
-                                                                  (8 * EMIT_CHUNK) * (8 * EMIT_CHUNK));
+                const uint32_t emitBlocks = xcdRunGrid((cells + BLOCK - 1) / BLOCK, EMIT_CHUNK);
                 kEmitWeighted<T><<<emitBlocks, BLOCK, 0, itsStream>>>(
                         cells, k0, wordsPerNode, static_cast<uint32_t>(cursor), itsNodeGrid.as<NodeGrid>(), bitmap,
                         prefix, itsCellKey.as<uint32_t>(), itsCellNode.as<uint32_t>(), itsCellStart.as<uint32_t>(),

@@ -37,6 +37,21 @@ constexpr uint32_t TILE = 256;  // inputs per workgroup in the per-input kernels
 // neighbourhood touches lie a few rows apart instead of a whole z-plane apart.
 constexpr uint32_t PENCIL = 4;
 constexpr uint32_t EMIT_CHUNK = 32;  // consecutive cell blocks of the weighted emit per XCD turn
+constexpr uint32_t XCD_TILE_RUN = 32;  // consecutive input tiles of kMark / kScatterPayload per XCD turn
+
+// Workgroup b runs on XCD b mod 8, each XCD with its own L2. The work item of workgroup b when
+// every XCD is to take runs of `run` consecutive items: rounds of 8 * run items, XCD x taking
+// items x * run .. x * run + run - 1 of each round. Grids are rounded up to whole rounds.
+__device__ __forceinline__ uint32_t xcdRunItem(uint32_t b, uint32_t run)
+{
+    return (b / (XCD_COUNT * run)) * (XCD_COUNT * run) + (b % XCD_COUNT) * run + (b / XCD_COUNT) % run;
+}
+
+__host__ __forceinline__ uint32_t xcdRunGrid(uint64_t items, uint32_t run)
+{
+    const uint64_t round = static_cast<uint64_t>(XCD_COUNT) * run;
+    return static_cast<uint32_t>((items + round - 1) / round * round);
+}
 
 __host__ __device__ __forceinline__ uint32_t pencilsPerAxis(uint32_t g)
 {
@@ -581,7 +596,8 @@ __device__ __forceinline__ void markTile(uint32_t tile, uint32_t nodeBase, uint6
 }
 
 template <typename T>
-static __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t nodeBase, uint64_t wordsPerNode,
+static __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t tiles, uint32_t nodeBase,
+                                               uint64_t wordsPerNode,
                                                const uint32_t* __restrict__ tileList,
                                                const uint32_t* __restrict__ segTile,
                                                const uint32_t* __restrict__ segCount,
@@ -591,7 +607,11 @@ static __global__ __launch_bounds__(TILE) void kMark(uint32_t tileBase, uint32_t
                                                uint32_t* __restrict__ bitmap, uint32_t* __restrict__ cellOf,
                                                uint32_t* __restrict__ err)
 {
-    markTile<T>(tileBase + blockIdx.x, nodeBase, wordsPerNode, tileList, segTile, segCount, segSrc, segInput,
+    // consecutive tiles share bitmap words: runs of them per XCD
+    const uint32_t t = xcdRunItem(blockIdx.x, XCD_TILE_RUN);
+    if (t >= tiles)
+        return;
+    markTile<T>(tileBase + t, nodeBase, wordsPerNode, tileList, segTile, segCount, segSrc, segInput,
                 nodeGrid, ps, bitmap, cellOf, err);
 }
 
@@ -837,7 +857,8 @@ __device__ __forceinline__ void scatterPayloadTile(uint32_t tile, const uint32_t
 }
 
 template <typename T, bool RANDOM>
-static __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, const uint32_t* __restrict__ tileList,
+static __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase, uint32_t tiles,
+                                                         const uint32_t* __restrict__ tileList,
                                                          const uint32_t* __restrict__ segTile,
                                                          const uint32_t* __restrict__ segCount,
                                                          const uint32_t* __restrict__ segSrc,
@@ -850,7 +871,11 @@ static __global__ __launch_bounds__(TILE) void kScatterPayload(uint32_t tileBase
                                                          WeightPayload* __restrict__ payload,
                                                          uint64_t* __restrict__ winKey)
 {
-    scatterPayloadTile<T, RANDOM>(tileBase + blockIdx.x, tileList, segTile, segCount, segSrc, segInput, nodeGrid,
+    // consecutive tiles store to neighbouring payload slots: runs of them per XCD
+    const uint32_t t = xcdRunItem(blockIdx.x, XCD_TILE_RUN);
+    if (t >= tiles)
+        return;
+    scatterPayloadTile<T, RANDOM>(tileBase + t, tileList, segTile, segCount, segSrc, segInput, nodeGrid,
                                   cellKey, rankOf, cellStart, cellOf, ps, payload, winKey);
 }
 
@@ -1119,11 +1144,8 @@ static __global__ __launch_bounds__(BLOCK) void kEmitWeighted(uint32_t cells, ui
                                                         const WeightPayload* __restrict__ payload,
                                                         const uint64_t* __restrict__ winKey, PointStore<T> ps)
 {
-    // workgroup b runs on XCD b mod 8; give each XCD runs of EMIT_CHUNK consecutive cell blocks
-    const uint32_t b     = blockIdx.x;
-    const uint32_t round = (b / 8) / EMIT_CHUNK;
-    const uint32_t block = round * 8 * EMIT_CHUNK + (b % 8) * EMIT_CHUNK + (b / 8) % EMIT_CHUNK;
-    const uint32_t c     = block * BLOCK + threadIdx.x;
+    // each XCD takes runs of EMIT_CHUNK consecutive cell blocks
+    const uint32_t c = xcdRunItem(blockIdx.x, EMIT_CHUNK) * BLOCK + threadIdx.x;
     if (c >= cells)
         return;
     emitWeightedCell<T>(c, nodeBase, wordsPerNode, cursor, nodeGrid, bitmap, prefix, cellKey, cellNode, cellStart,
