@@ -123,6 +123,8 @@ private:
     template <typename T>
     void countPoints();
     void mergeLevels();
+    // whether the fine sort writes the level-0 occupancy bits (a bucket's level-0 rows are 32 cells)
+    bool levelZeroBits() const;
     template <typename T>
     void distributePoints();
     template <typename T>
@@ -180,7 +182,7 @@ private:
     DeviceBuffer itsCloudOwned;
 
     // dense pyramid and sparse nodes
-    DeviceBuffer itsCount, itsDenseToSparse;
+    DeviceBuffer itsCount, itsDenseToSparse, itsOccupancy;  // dense counts, sparse ids, occupancy bits
     DeviceBuffer itsNodeKey, itsNodeCount, itsNodePoints, itsNodeData, itsNodeParent, itsNodeChild, itsNodeFlags,
             itsNodeSorted, itsNodeAux, itsNodeStage, itsNodeSub, itsNodeRegion;
 

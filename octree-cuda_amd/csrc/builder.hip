@@ -192,6 +192,7 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
     // and the output region.
     itsCount.ensure(sizeof(uint32_t) * static_cast<uint64_t>(itsDense));
     itsDenseToSparse.ensure(sizeof(int32_t) * static_cast<uint64_t>(itsDense));
+    itsOccupancy.ensure(sizeof(uint32_t) * ((static_cast<uint64_t>(itsDense) + 31) / 32));
     uint64_t mBound = std::min<uint64_t>(itsDense, static_cast<uint64_t>(itsDepth + 1) * cloud.pointAmount);
     // The previous build's snapshot copies may still read the node keys and counts, the LUT and
     // the output: growing one of them frees it, so those copies are waited for first. (The other
@@ -236,7 +237,7 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
 
 std::vector<DeviceBuffer*> OctreeBuilder::arena()
 {
-    return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsNodeKey,      &itsNodeCount,   &itsNodePoints,
+    return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsOccupancy,    &itsNodeKey,      &itsNodeCount,   &itsNodePoints,
             &itsNodeData,    &itsNodeParent,  &itsNodeChild,     &itsNodeFlags,    &itsNodeSorted,  &itsNodeAux,
             &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
             &itsPerm,
@@ -281,7 +282,7 @@ uint64_t OctreeBuilder::estimateFootprint(const CloudInfo& cloud, const Processi
     const uint64_t staged = e > n ? e - n : 1;
     const uint64_t mBound = std::min<uint64_t>(dense, (depth + 1) * n);
     uint64_t bytes        = hostCloud ? n * cloud.dataStride : 0;
-    bytes += 8 * dense;                                     // count pyramid, dense -> sparse
+    bytes += 8 * dense + 4 * ((dense + 31) / 32);          // count pyramid, dense -> sparse, occupancy bits
     bytes += 8 * mBound + 77 * mBound + n / 1024;           // node keys / counts, node arrays, leaf chunks
     bytes += 22 * e;                                        // lut + output records
     bytes += (2 * sortedPointBytes(cloud) + 2 + 4) * n;     // bucketed, sorted, fine codes, permutation
@@ -506,8 +507,10 @@ void OctreeBuilder::countPoints()
         fp.levelOffset[l] = itsLevelOffset[l];
         fp.levelGrid[l]   = itsLevelGrid[l];
     }
+    // the level-0 occupancy bits with the counts where a bucket row is a whole bitmap word
+    uint32_t* occ0 = levelZeroBits() ? itsOccupancy.as<uint32_t>() : nullptr;
     kFineSort<<<nb, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart, fp,
-                                                  itsCount.as<uint32_t>(), itsPerm.as<uint32_t>());
+                                                  itsCount.as<uint32_t>(), itsPerm.as<uint32_t>(), occ0);
     OC_LAUNCH_CHECK();
     if (n > FINE_SPLIT)
     {
@@ -525,7 +528,8 @@ void OctreeBuilder::countPoints()
                                                              itsSplitHist.as<uint32_t>());
         OC_LAUNCH_CHECK();
         kFineSplitScan<<<grid, SORT_THREADS, 0, itsStream>>>(bucketStart, itsSplitChunks.as<uint64_t>(), chunkCount,
-                                                             fp, itsCount.as<uint32_t>(), itsSplitHist.as<uint32_t>());
+                                                             fp, itsCount.as<uint32_t>(), itsSplitHist.as<uint32_t>(),
+                                                             occ0);
         OC_LAUNCH_CHECK();
         kFineChunkPerm<<<grid, SORT_THREADS, 0, itsStream>>>(itsFineCode.as<uint16_t>(), bucketStart,
                                                              itsSplitChunks.as<uint64_t>(), chunkCount, fp.lc,
@@ -533,6 +537,12 @@ void OctreeBuilder::countPoints()
         OC_LAUNCH_CHECK();
     }
     OC_HIP_CHECK(hipEventRecord(itsEvents[EV_COUNT_END], itsStream));  // the records move with the distribution
+}
+
+bool OctreeBuilder::levelZeroBits() const
+{
+    // 32-cell rows starting at multiples of 32, and level 1 starting at a multiple of 64
+    return itsCoarse == MAX_COARSE_LEVEL && itsProc.chunkingGrid % 64 == 0;
 }
 
 void OctreeBuilder::mergeLevels()
@@ -546,15 +556,22 @@ void OctreeBuilder::mergeLevels()
         OC_LAUNCH_CHECK();
     }
 
-    // sparse ids (A2) + node keys / counts + first id of every level
+    // occupancy bits of the levels the fine sort has not marked, then sparse ids (A2) + node keys /
+    // counts + first id of every level, by one scan over the bitmap's words
+    uint32_t* occ         = itsOccupancy.as<uint32_t>();
+    const uint64_t marked = levelZeroBits() ? itsLevelOffset[1] : 0;
+    kOccupancyBits<<<gridFor(itsDense - marked), BLOCK, 0, itsStream>>>(count, marked, itsDense, occ);
+    OC_LAUNCH_CHECK();
     uint32_t* levelStart = reinterpret_cast<uint32_t*>(sc + SC_LEVEL_START);
-    itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(itsDense) + 1));
-    OccupiedOut occOut{};
+    const uint64_t words = (static_cast<uint64_t>(itsDense) + 31) / 32;
+    itsScanScratch.ensure(sizeof(uint64_t) * (scanTiles(words) + 1));
+    OccupiedWordOut occOut{};
+    occOut.occ           = occ;
     occOut.denseToSparse = itsDenseToSparse.as<int32_t>();
     occOut.nodeKey       = itsNodeKey.as<uint32_t>();
     occOut.nodeCount     = itsNodeCount.as<uint32_t>();
     occOut.count         = count;
-    scanExclusive<uint32_t>(itsDense, OccupiedIn{count}, occOut, itsScanScratch.as<uint32_t>(),
+    scanExclusive<uint32_t>(words, OccupiedWordIn{occ}, occOut, itsScanScratch.as<uint32_t>(),
                             reinterpret_cast<uint32_t*>(sc + SC_M), itsStream);
     OC_LAUNCH_CHECK();
     LevelOffsets lo{};

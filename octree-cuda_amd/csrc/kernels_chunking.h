@@ -71,6 +71,51 @@ static __global__ __launch_bounds__(BLOCK) void kPropagate(uint32_t* __restrict_
         propagateOne(static_cast<uint32_t>(i), count, offset, grid, childOffset, childGrid);
 }
 
+// Occupancy bitmap of the dense cells [from, dense) from their counts (bit d: cell d holds
+// points). `from` is a multiple of 64 and every wave covers 64 consecutive cells, two whole words.
+static __global__ __launch_bounds__(BLOCK) void kOccupancyBits(const uint32_t* __restrict__ count, uint64_t from,
+                                                         uint64_t dense, uint32_t* __restrict__ occ)
+{
+    const uint64_t n = (dense - from + 63) / 64 * 64;  // whole waves: the loop is uniform per wave
+    for (uint64_t i = globalThread(); i < n; i += globalStride())
+    {
+        const uint64_t d    = from + i;
+        const uint64_t bits = __ballot(d < dense && count[d] != 0);
+        if ((threadIdx.x & 31) == 0 && d < dense)
+            occ[d >> 5] = static_cast<uint32_t>(bits >> (threadIdx.x & 32));
+    }
+}
+
+// Sparse ids (A2) from the occupancy bitmap, one scan item per word: the exclusive rank of an
+// occupied dense cell; its node key and dense count are written in the same pass.
+struct OccupiedWordIn
+{
+    const uint32_t* occ;
+    __device__ uint32_t operator()(uint64_t w) const { return __popc(occ[w]); }
+};
+
+struct OccupiedWordOut
+{
+    const uint32_t* occ;
+    int32_t* denseToSparse;
+    uint32_t* nodeKey;
+    uint32_t* nodeCount;
+    const uint32_t* count;
+    __device__ void operator()(uint64_t w, uint32_t prefix, uint32_t v) const
+    {
+        uint32_t bits = v ? occ[w] : 0u;
+        while (bits)
+        {
+            const uint64_t d  = 32 * w + static_cast<uint32_t>(__builtin_ctz(bits));
+            bits &= bits - 1u;
+            denseToSparse[d]  = static_cast<int32_t>(prefix);
+            nodeKey[prefix]   = static_cast<uint32_t>(d);
+            nodeCount[prefix] = count[d];
+            ++prefix;
+        }
+    }
+};
+
 struct OccupiedIn
 {
     const uint32_t* count;

@@ -320,9 +320,11 @@ struct FineSortParams
 
 // Dense counts of bucket b's cells on levels 0..lc (kernelPropagatePointCounts,
 // point_count_propagation.cuh:34-86, as differences of the Morton prefix h over the bucket's
-// fine cells; h null: an empty bucket, all zero).
+// fine cells; h null: an empty bucket, all zero). With occ and 32-cell rows (lc = 5), also the
+// level-0 occupancy bits: a wave writes two whole rows, 32 consecutive cells from a multiple of
+// 32, so each row is one word of the bitmap, written whole, no atomics.
 __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& fp, const uint32_t* h,
-                                             uint32_t* __restrict__ count)
+                                             uint32_t* __restrict__ count, uint32_t* __restrict__ occ = nullptr)
 {
     const uint32_t bx = compact3(b >> 2), by = compact3(b >> 1), bz = compact3(b);
     for (uint32_t k = 0; k <= fp.lc; ++k)
@@ -342,7 +344,14 @@ __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& f
             }
             const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
                            gz = static_cast<uint64_t>(bz) * side + lz;
-            count[offset + gx + gy * gk + gz * gk * gk] = v;
+            const uint64_t d = offset + gx + gy * gk + gz * gk * gk;
+            count[d]         = v;
+            if (k == 0 && occ && side == 32)  // uniform; every lane of the wave is here (n3 % SORT_THREADS == 0)
+            {
+                const uint64_t bits = __ballot(v != 0);
+                if ((threadIdx.x & 31) == 0)
+                    occ[d >> 5] = static_cast<uint32_t>(bits >> (threadIdx.x & 32));
+            }
         }
     }
 }
@@ -455,7 +464,7 @@ __device__ __forceinline__ void finePermute(const uint16_t* __restrict__ fineCod
 __device__ __forceinline__ void fineSortBucket(uint32_t b, const uint16_t* __restrict__ fineCode,
                                                const uint32_t* __restrict__ bucketStart, const FineSortParams& fp,
                                                uint32_t* __restrict__ count, uint32_t* __restrict__ perm, uint32_t* h,
-                                               uint32_t* scratch)
+                                               uint32_t* scratch, uint32_t* __restrict__ occ = nullptr)
 {
     const uint32_t cells = 1u << (3 * fp.lc);
     const uint32_t lo = bucketStart[b], hi = bucketStart[b + 1];
@@ -464,7 +473,7 @@ __device__ __forceinline__ void fineSortBucket(uint32_t b, const uint16_t* __res
     if (lo == hi)
     {
         // an empty bucket (most of them in a sparse cloud): zero counts, no histogram, no scan
-        bucketCounts(b, fp, nullptr, count);
+        bucketCounts(b, fp, nullptr, count, occ);
         return;
     }
 
@@ -474,7 +483,7 @@ __device__ __forceinline__ void fineSortBucket(uint32_t b, const uint16_t* __res
     fineHistogram(fineCode, lo, hi, h);
     __syncthreads();
     bucketScan(h, cells, scratch);
-    bucketCounts(b, fp, h, count);
+    bucketCounts(b, fp, h, count, occ);
     __syncthreads();
 
     // the permutation into Morton order: 4-byte scattered stores instead of 32-byte records
@@ -483,11 +492,12 @@ __device__ __forceinline__ void fineSortBucket(uint32_t b, const uint16_t* __res
 
 static __global__ __launch_bounds__(SORT_THREADS) void kFineSort(const uint16_t* __restrict__ fineCode,
                                                            const uint32_t* __restrict__ bucketStart, FineSortParams fp,
-                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm)
+                                                           uint32_t* __restrict__ count, uint32_t* __restrict__ perm,
+                                                           uint32_t* __restrict__ occ)
 {
     __shared__ uint32_t h[MAX_FINE_CELLS + 1];
     __shared__ uint32_t scratch[SORT_WAVES];
-    fineSortBucket(blockIdx.x, fineCode, bucketStart, fp, count, perm, h, scratch);
+    fineSortBucket(blockIdx.x, fineCode, bucketStart, fp, count, perm, h, scratch, occ);
 }
 
 // chunks of the buckets above FINE_SPLIT points: entry = first chunk of the bucket's points,
@@ -556,7 +566,7 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineChunkHist(const uint
 __device__ __forceinline__ void fineSplitScan(uint32_t chunk, const uint32_t* __restrict__ bucketStart,
                                               const uint64_t* __restrict__ chunks, const FineSortParams& fp,
                                               uint32_t* __restrict__ count, uint32_t* __restrict__ chunkHist,
-                                              uint32_t* h, uint32_t* scratch)
+                                              uint32_t* h, uint32_t* scratch, uint32_t* __restrict__ occ = nullptr)
 {
     const uint32_t b     = static_cast<uint32_t>(chunks[chunk] >> 32);
     const uint32_t parts = (bucketStart[b + 1] - bucketStart[b] + FINE_SPLIT - 1) / FINE_SPLIT;
@@ -571,7 +581,7 @@ __device__ __forceinline__ void fineSplitScan(uint32_t chunk, const uint32_t* __
     }
     __syncthreads();
     bucketScan(h, cells, scratch);  // every chunk histogram has been read
-    bucketCounts(b, fp, h, count);
+    bucketCounts(b, fp, h, count, occ);
     for (uint32_t c = threadIdx.x; c < cells; c += SORT_THREADS)
         first[c] = h[c];
     for (uint32_t k = 1 + threadIdx.x; k < parts; k += SORT_THREADS)
@@ -595,13 +605,14 @@ static __global__ __launch_bounds__(SORT_THREADS) void kFineSplitScan(const uint
                                                                 const uint64_t* __restrict__ chunks,
                                                                 const uint32_t* __restrict__ chunkCount,
                                                                 FineSortParams fp, uint32_t* __restrict__ count,
-                                                                uint32_t* __restrict__ chunkHist)
+                                                                uint32_t* __restrict__ chunkHist,
+                                                                uint32_t* __restrict__ occ)
 {
     __shared__ uint32_t h[MAX_FINE_CELLS + 1];
     __shared__ uint32_t scratch[SORT_WAVES];
     if (blockIdx.x >= *chunkCount || static_cast<uint32_t>(chunks[blockIdx.x]) != 0)
         return;  // the whole workgroup: not the first chunk of a bucket
-    fineSplitScan(blockIdx.x, bucketStart, chunks, fp, count, chunkHist, h, scratch);
+    fineSplitScan(blockIdx.x, bucketStart, chunks, fp, count, chunkHist, h, scratch, occ);
 }
 
 // Split path, 3: partition `part` of the bucket's fine cells: every point of the bucket is read,
