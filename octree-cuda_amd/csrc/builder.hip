@@ -623,7 +623,7 @@ void OctreeBuilder::mergeLevels()
             itsM, lp, itsNodeKey.as<uint32_t>(), itsNodeCount.as<uint32_t>(), count, itsDenseToSparse.as<int32_t>(),
             itsNodePoints.as<uint32_t>(), itsNodeData.as<uint64_t>(), itsNodeParent.as<int32_t>(),
             itsNodeChild.as<int32_t>(), itsNodeFlags.as<uint8_t>(), itsNodeAux.as<uint32_t>(),
-            itsNodeSub.as<uint32_t>());
+            itsNodeSub.as<uint32_t>(), occ);
     OC_LAUNCH_CHECK();
 
     // leaf data blocks (A4): exclusive scan of the block sizes in id order

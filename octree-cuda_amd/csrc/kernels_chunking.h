@@ -205,7 +205,7 @@ __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, 
                                             uint32_t* __restrict__ pointCount, uint64_t* __restrict__ dataIdx,
                                             int32_t* __restrict__ parent, int32_t* __restrict__ children,
                                             uint8_t* __restrict__ flags, uint32_t* __restrict__ blockSize,
-                                            uint32_t* __restrict__ subtree)
+                                            uint32_t* __restrict__ subtree, const uint32_t* __restrict__ occ = nullptr)
 {
     {
         uint32_t key = nodeKey[id];
@@ -232,7 +232,10 @@ __device__ __forceinline__ void nodeLinksOf(uint32_t id, const LevelParams& lp, 
             {
                 uint32_t cx = 2 * x + ((k >> 2) & 1), cy = 2 * y + ((k >> 1) & 1), cz = 2 * z + (k & 1);
                 const uint32_t cd = lp.offset[l - 1] + cx + cy * cg + cz * cg * cg;
-                int32_t c         = denseCount[cd] ? denseToSparse[cd] : -1;  // set for occupied cells only
+                // occupancy from the bitmap where there is one (level-0 counts of empty cells are
+                // not written then); denseToSparse is set for occupied cells only
+                const bool occupied = occ ? ((occ[cd >> 5] >> (cd & 31)) & 1u) != 0 : denseCount[cd] != 0;
+                int32_t c           = occupied ? denseToSparse[cd] : -1;
                 children[8 * static_cast<uint64_t>(id) + k] = c;
                 if (internal && c >= 0)
                 {
@@ -265,11 +268,11 @@ static __global__ __launch_bounds__(BLOCK) void kNodeLinks(uint32_t m, LevelPara
                                                      int32_t* __restrict__ parent,
                                                      int32_t* __restrict__ children, uint8_t* __restrict__ flags,
                                                      uint32_t* __restrict__ blockSize,
-                                                     uint32_t* __restrict__ subtree)
+                                                     uint32_t* __restrict__ subtree, const uint32_t* __restrict__ occ)
 {
     for (uint64_t i = globalThread(); i < m; i += globalStride())
         nodeLinksOf(static_cast<uint32_t>(i), lp, nodeKey, nodeCount, denseCount, denseToSparse, pointCount, dataIdx,
-                    parent, children, flags, blockSize, subtree);
+                    parent, children, flags, blockSize, subtree, occ);
 }
 
 struct ArrayIn

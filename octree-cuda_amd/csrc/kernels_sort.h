@@ -320,9 +320,11 @@ struct FineSortParams
 
 // Dense counts of bucket b's cells on levels 0..lc (kernelPropagatePointCounts,
 // point_count_propagation.cuh:34-86, as differences of the Morton prefix h over the bucket's
-// fine cells; h null: an empty bucket, all zero). With occ and 32-cell rows (lc = 5), also the
-// level-0 occupancy bits: a wave writes two whole rows, 32 consecutive cells from a multiple of
-// 32, so each row is one word of the bitmap, written whole, no atomics.
+// fine cells; h null: an empty bucket, all zero). With occ and 32-cell rows (lc = 5), the
+// level-0 occupancy bits instead of the level-0 zero counts: a wave writes two whole rows, 32
+// consecutive cells from a multiple of 32, so each row is one word of the bitmap, written whole,
+// no atomics; the count of a level-0 cell is then written only where it is not zero (the later
+// passes read level-0 counts of occupied cells only, and test occupancy on the bitmap).
 __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& fp, const uint32_t* h,
                                              uint32_t* __restrict__ count, uint32_t* __restrict__ occ = nullptr)
 {
@@ -333,6 +335,18 @@ __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& f
         const uint32_t n3     = side * side * side;
         const uint64_t gk     = fp.levelGrid[k];
         const uint64_t offset = fp.levelOffset[k];
+        const bool bits       = k == 0 && occ && side == 32;
+        if (bits && !h)
+        {
+            // an empty bucket: its 32 x 32 rows of level 0 are zero words
+            for (uint32_t r = threadIdx.x; r < side * side; r += SORT_THREADS)
+            {
+                const uint64_t gy = static_cast<uint64_t>(by) * side + (r % side),
+                               gz = static_cast<uint64_t>(bz) * side + (r / side);
+                occ[(offset + static_cast<uint64_t>(bx) * side + gy * gk + gz * gk * gk) >> 5] = 0;
+            }
+            continue;
+        }
         for (uint32_t j = threadIdx.x; j < n3; j += SORT_THREADS)
         {
             const uint32_t lx = j & (side - 1), ly = (j >> (fp.lc - k)) & (side - 1), lz = j >> (2 * (fp.lc - k));
@@ -345,12 +359,13 @@ __device__ __forceinline__ void bucketCounts(uint32_t b, const FineSortParams& f
             const uint64_t gx = static_cast<uint64_t>(bx) * side + lx, gy = static_cast<uint64_t>(by) * side + ly,
                            gz = static_cast<uint64_t>(bz) * side + lz;
             const uint64_t d = offset + gx + gy * gk + gz * gk * gk;
-            count[d]         = v;
-            if (k == 0 && occ && side == 32)  // uniform; every lane of the wave is here (n3 % SORT_THREADS == 0)
+            if (!bits || v)
+                count[d] = v;
+            if (bits)  // uniform; every lane of the wave is here (n3 % SORT_THREADS == 0)
             {
-                const uint64_t bits = __ballot(v != 0);
+                const uint64_t set = __ballot(v != 0);
                 if ((threadIdx.x & 31) == 0)
-                    occ[d >> 5] = static_cast<uint32_t>(bits >> (threadIdx.x & 32));
+                    occ[d >> 5] = static_cast<uint32_t>(set >> (threadIdx.x & 32));
             }
         }
     }
