@@ -241,3 +241,23 @@ def test_points_outside_the_box_fail_the_build():
         compare(fetch(s), orc)
     finally:
         s.close()
+
+
+def test_rebuild_on_one_session_with_other_clouds():
+    """Clouds of other shapes and precisions built one after the other on one session at chunking
+    grid 512, where the fine sort writes the level-0 occupancy bits and only the level-0 counts that
+    are not zero: cells that one cloud occupies and the next leaves empty keep the earlier counts
+    (and leaf ids), which no pass of the next build may read. Each build equals the oracle's."""
+    s = api.Session(0)
+    try:
+        for n, cloud_type, box, seed, mode in ((150_000, 0, (100.0, 100.0, 20.0), 31, MODES[2]),
+                                                (120_000, 1, (50.0, 80.0, 60.0), 32, MODES[6]),
+                                                (150_000, 0, (100.0, 100.0, 20.0), 33, MODES[5])):
+            print(f"case rebuild G=512 g=128 T=1000 n={n} cloud_type={cloud_type} seed={seed} mode={mode}", flush=True)
+            cloud, spec, cubic = synthetic(n, cloud_type, box=box, clusters=4, seed=seed)
+            build = _spec(512, 128, 1000, mode, F=6.0)  # these clouds need N + I = 3.7 N
+            gpu_build(cloud, spec.points, spec.stride, cloud_type, cubic, [spec.scale] * 3, build, sess=s)
+            orc = O.build(cloud, spec.points, spec.stride, cloud_type, cubic, [spec.scale] * 3, build)
+            compare(fetch(s), orc)
+    finally:
+        s.close()
