@@ -237,7 +237,8 @@ void OctreeBuilder::setup(const CloudInfo& cloud, const ProcessingInfo& proc, co
 
 std::vector<DeviceBuffer*> OctreeBuilder::arena()
 {
-    return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsOccupancy,    &itsNodeKey,      &itsNodeCount,   &itsNodePoints,
+    return {&itsCloudOwned,  &itsCount,       &itsDenseToSparse, &itsOccupancy,
+            &itsNodeKey,     &itsNodeCount,   &itsNodePoints,
             &itsNodeData,    &itsNodeParent,  &itsNodeChild,     &itsNodeFlags,    &itsNodeSorted,  &itsNodeAux,
             &itsNodeStage,   &itsNodeSub,     &itsNodeRegion,    &itsBucketed,     &itsSorted,      &itsFineCode,
             &itsPerm,
